@@ -1,0 +1,1075 @@
+// =============================================================================
+//  ORACLE -- test infrastructure only.
+//
+//  CPU restatement of BROADCAST's FP64 residual / tangent path, used as the
+//  checker for the HIP product (broadcast_amd) and as the "port" CPU
+//  baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+//  leg may load this library.  It is never part of the product path.
+//
+//  Every function follows the expanded reference Fortran statement by
+//  statement, in the same floating-point operation order, so that on the same
+//  inputs it agrees with the flang-built reference (oracle/_ref) to the last
+//  bit wherever libm agrees.  The tangent is obtained by instantiating the same
+//  templates on a forward-mode dual number whose non-smooth rules are
+//  TAPENADE's (srcfv/tangent/flux_num_dnc5_d.f90):
+//    abs(x)      -> branch x >= 0                      (:945-951)
+//    sqrt(x)     -> derivative 0 when x == 0           (:953-957)
+//    max(a,b)    -> IF (a < b) THEN b ELSE a           (:1054-1074, :1117-1123)
+//    tanh        -> analytic                            (:1025-1026)
+//    sign(h,x)   -> piecewise constant                 (bc_*_d.f90)
+//
+//  Array convention: Fortran column-major with lower bound 1-gh, exactly the
+//  reference's explicit-shape dummies (srcfv/rhs/flux_num_dnc5.F90:17-26).
+// =============================================================================
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+// ---------------------------------------------------------------- dual number
+struct Dual {
+  double v, d;
+  Dual() : v(0), d(0) {}
+  Dual(double a) : v(a), d(0) {}
+  Dual(double a, double b) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) {
+  double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator+(double a, Dual b) { return Dual(a) + b; }
+inline Dual operator-(double a, Dual b) { return Dual(a) - b; }
+inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+inline Dual operator/(double a, Dual b) { return Dual(a) / b; }
+inline Dual operator+(Dual a, double b) { return a + Dual(b); }
+inline Dual operator-(Dual a, double b) { return a - Dual(b); }
+inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>=(Dual a, double b) { return a.v >= b; }
+
+inline double val(double a) { return a; }
+inline double val(Dual a) { return a.v; }
+
+inline double f_sqrt(double a) { return std::sqrt(a); }
+inline Dual f_sqrt(Dual a) {
+  double s = std::sqrt(a.v);
+  return {s, a.v == 0.0 ? 0.0 : a.d / (2.0 * s)};
+}
+inline double f_tanh(double a) { return std::tanh(a); }
+inline Dual f_tanh(Dual a) {
+  double t = std::tanh(a.v);
+  return {t, (1.0 - t * t) * a.d};
+}
+template <class T> inline T f_abs(T a) { return val(a) >= 0.0 ? a : -a; }
+template <class T> inline T f_max(T a, T b) { return val(a) < val(b) ? b : a; }
+inline double f_pow(double a, double e) { return std::pow(a, e); }
+inline Dual f_pow(Dual a, double e) {
+  double p = std::pow(a.v, e);
+  return {p, a.v == 0.0 ? 0.0 : e * std::pow(a.v, e - 1.0) * a.d};
+}
+// Fortran SIGN(a, b) with a > 0: |a| if b >= 0 (sign bit clear), else -|a|.
+inline double f_sign(double a, double b) { return std::copysign(std::fabs(a), b); }
+
+// ------------------------------------------------------- Fortran-style views
+template <class T> struct F2 {
+  T* p; int lo1, n1, lo2;
+  T& operator()(int i, int j) const { return p[(i - lo1) + (long)n1 * (j - lo2)]; }
+};
+template <class T> struct F3 {
+  T* p; int lo1, n1, lo2, n2, lo3;
+  T& operator()(int i, int j, int k) const {
+    return p[(i - lo1) + (long)n1 * ((j - lo2) + (long)n2 * (k - lo3))];
+  }
+};
+template <class T> struct F4 {
+  T* p; int lo1, n1, lo2, n2, n3;
+  T& operator()(int i, int j, int k, int l) const {  // k, l 1-based
+    return p[(i - lo1) + (long)n1 * ((j - lo2) + (long)n2 * ((k - 1) + (long)n3 * (l - 1)))];
+  }
+};
+
+struct Phys {
+  double cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4;
+};
+
+// =============================================================================
+//  Residual: flux_num_dnc5_2d  (srcfv/rhs/flux_num_dnc5.F90:7-226; expanded
+//  form srcfv/prepro/flux_num_dnc5.f90)
+// =============================================================================
+template <class T>
+void residual(T* residu_p, const T* w_p, const double* nx_p, const double* ny_p,
+              const double* vol_p, const double* volf_p, int gh, const Phys& ph,
+              int im, int jm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh;
+  const int lo = 1 - gh;
+  F3<T> residu{residu_p, lo, ni, lo, nj, 1};
+  F3<const T> w{w_p, lo, ni, lo, nj, 1};
+  F3<const double> nx{nx_p, lo, ni + 1, lo, nj + 1, 1};
+  F3<const double> ny{ny_p, lo, ni + 1, lo, nj + 1, 1};
+  F2<const double> vol{vol_p, lo, ni, lo};
+  F3<const double> volf{volf_p, lo, ni, lo, nj, 1};
+
+  const long ncell = (long)ni * nj;
+  std::vector<T> hn_v((long)(ni + 1) * (nj + 1) * 10), f_v(ncell * 5), g_v(ncell * 5);
+  std::vector<T> velx_v(ncell), vely_v(ncell), velz_v(ncell), tloc_v(ncell), p_v(ncell), mu_v(ncell);
+  std::vector<T> gradu_v(ncell * 2), gradv_v(ncell * 2);
+  F4<T> hn{hn_v.data(), lo, ni + 1, lo, nj + 1, 5};
+  F3<T> f{f_v.data(), lo, ni, lo, nj, 1}, g{g_v.data(), lo, ni, lo, nj, 1};
+  F2<T> velx{velx_v.data(), lo, ni, lo}, vely{vely_v.data(), lo, ni, lo},
+      velz{velz_v.data(), lo, ni, lo}, tloc{tloc_v.data(), lo, ni, lo},
+      p{p_v.data(), lo, ni, lo}, mu{mu_v.data(), lo, ni, lo};
+  F3<T> gradu{gradu_v.data(), lo, ni, lo, nj, 1}, gradv{gradv_v.data(), lo, ni, lo, nj, 1};
+
+  const double HALF = 0.5, ONE = 1.0, ZERO = 0.0, TWO = 2.0;
+  const double TWOTHIRD = 2.0 / 3.0, FOURTH = 0.25, TWELFTH = 0.25 / 3.0;
+  const double cpprandtl = ph.cp / ph.prandtl;
+  const double cvm1 = ONE / ph.cv;
+  const double TWENTYFOURTH = ONE / 24.0;
+  const double ccross = TWELFTH * 0.0625;
+  double denom = 1.0 / 12.0;
+  const double b1 = 8.0 * denom, b2 = -denom;
+  denom = 1.0 / 60.0;
+  const double c1 = 37.0 * denom, c2 = -8.0 * denom, c3 = denom;
+  const double d1 = 10.0 * denom, d2 = 5.0 * denom, d3 = denom;
+  const double betas = ph.muref * (ph.tref + ph.cs) / (std::sqrt(ph.tref) * ph.tref);
+  const double gam = ph.gam, rgaz = ph.rgaz, k2 = ph.k2, k4 = ph.k4, s_suth = ph.s_suth;
+
+  // primvisc.F: every cell including ghosts (prepro lines 82-107)
+  for (int j = 1 - gh; j <= jm + gh; ++j)
+    for (int i = 1 - gh; i <= im + gh; ++i) {
+      T ro = w(i, j, 1);
+      T rom1 = ONE / ro;
+      velx(i, j) = w(i, j, 2) * rom1;
+      vely(i, j) = w(i, j, 3) * rom1;
+      velz(i, j) = w(i, j, 4) * rom1;
+      T ec = HALF * (velx(i, j) * velx(i, j) + vely(i, j) * vely(i, j) + velz(i, j) * velz(i, j));
+      T eloc = (w(i, j, 5) - ec * ro) * rom1;
+      tloc(i, j) = eloc * cvm1;
+      p(i, j) = (gam - ONE) * ro * eloc;
+      T htot = (w(i, j, 5) + p(i, j)) * rom1;
+      f(i, j, 1) = w(i, j, 2);
+      f(i, j, 2) = w(i, j, 2) * velx(i, j) + p(i, j);
+      f(i, j, 3) = w(i, j, 2) * vely(i, j);
+      f(i, j, 4) = w(i, j, 2) * velz(i, j);
+      f(i, j, 5) = w(i, j, 2) * htot;
+      g(i, j, 1) = w(i, j, 3);
+      g(i, j, 2) = w(i, j, 3) * velx(i, j);
+      g(i, j, 3) = w(i, j, 3) * vely(i, j) + p(i, j);
+      g(i, j, 4) = w(i, j, 3) * velz(i, j);
+      g(i, j, 5) = w(i, j, 3) * htot;
+      mu(i, j) = betas / (tloc(i, j) + s_suth) * f_sqrt(tloc(i, j)) * tloc(i, j);
+    }
+
+  // gradop_5pi/j + gradient.F on the interior (prepro lines 108-140); only
+  // gradu and gradv feed the residual (gradw/gradmu are dead).
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i) {
+      T gui = (b1 * (velx(i + 1, j) - velx(i - 1, j)) + b2 * (velx(i + 2, j) - velx(i - 2, j)));
+      T gvi = (b1 * (vely(i + 1, j) - vely(i - 1, j)) + b2 * (vely(i + 2, j) - vely(i - 2, j)));
+      T guj = (b1 * (velx(i, j + 1) - velx(i, j - 1)) + b2 * (velx(i, j + 2) - velx(i, j - 2)));
+      T gvj = (b1 * (vely(i, j + 1) - vely(i, j - 1)) + b2 * (vely(i, j + 2) - vely(i, j - 2)));
+      double volm1 = ONE / vol(i, j);
+      double dxm1 = HALF * (nx(i, j, 1) + nx(i + 1, j, 1)) * volm1;
+      double dxm2 = HALF * (nx(i, j, 2) + nx(i, j + 1, 2)) * volm1;
+      double dym1 = HALF * (ny(i, j, 1) + ny(i + 1, j, 1)) * volm1;
+      double dym2 = HALF * (ny(i, j, 2) + ny(i, j + 1, 2)) * volm1;
+      gradu(i, j, 1) = dxm1 * gui + dxm2 * guj;
+      gradv(i, j, 1) = dxm1 * gvi + dxm2 * gvj;
+      gradu(i, j, 2) = dym1 * gui + dym2 * guj;
+      gradv(i, j, 2) = dym1 * gvi + dym2 * gvj;
+    }
+  // gradveloingh.F (prepro lines 141-156): linear extrapolation into the ghost
+  // frame, rows first then columns, the whole sweep done twice.
+  for (int rep = 0; rep < 2; ++rep)
+    for (int h = 1; h <= gh; ++h) {
+      for (int k = 1; k <= 2; ++k)
+        for (int i = 1 - gh; i <= im + gh; ++i) {
+          gradu(i, 1 - h, k) = TWO * gradu(i, 2 - h, k) - gradu(i, 3 - h, k);
+          gradv(i, 1 - h, k) = TWO * gradv(i, 2 - h, k) - gradv(i, 3 - h, k);
+          gradu(i, jm + h, k) = TWO * gradu(i, jm - 1 + h, k) - gradu(i, jm - 2 + h, k);
+          gradv(i, jm + h, k) = TWO * gradv(i, jm - 1 + h, k) - gradv(i, jm - 2 + h, k);
+        }
+      for (int k = 1; k <= 2; ++k)
+        for (int j = 1 - gh; j <= jm + gh; ++j) {
+          gradu(1 - h, j, k) = TWO * gradu(2 - h, j, k) - gradu(3 - h, j, k);
+          gradv(1 - h, j, k) = TWO * gradv(2 - h, j, k) - gradv(3 - h, j, k);
+          gradu(im + h, j, k) = TWO * gradu(im - 1 + h, j, k) - gradu(im - 2 + h, j, k);
+          gradv(im + h, j, k) = TWO * gradv(im - 1 + h, j, k) - gradv(im - 2 + h, j, k);
+        }
+    }
+
+  // ------------------------------------------------------ face building blocks
+  // euler_o6_i.F: centred 6-point flux through the i-face (i,j).
+  auto euler_i = [&](int i, int j, T fx[5]) {
+    for (int e = 1; e <= 5; ++e)
+      fx[e - 1] = (c1 * (f(i, j, e) + f(i - 1, j, e)) + c2 * (f(i + 1, j, e) + f(i - 2, j, e)) +
+                   c3 * (f(i + 2, j, e) + f(i - 3, j, e))) * nx(i, j, 1) +
+                  (c1 * (g(i, j, e) + g(i - 1, j, e)) + c2 * (g(i + 1, j, e) + g(i - 2, j, e)) +
+                   c3 * (g(i + 2, j, e) + g(i - 3, j, e))) * ny(i, j, 1);
+  };
+  auto euler_j = [&](int i, int j, T fx[5]) {
+    for (int e = 1; e <= 5; ++e)
+      fx[e - 1] = (c1 * (f(i, j, e) + f(i, j - 1, e)) + c2 * (f(i, j + 1, e) + f(i, j - 2, e)) +
+                   c3 * (f(i, j + 2, e) + f(i, j - 3, e))) * nx(i, j, 2) +
+                  (c1 * (g(i, j, e) + g(i, j - 1, e)) + c2 * (g(i, j + 1, e) + g(i, j - 2, e)) +
+                   c3 * (g(i, j + 2, e) + g(i, j - 3, e))) * ny(i, j, 2);
+  };
+  // nearbndfluxes{5,3}demi_7p.F with coefnearbnd_7p.F: off-centred j-faces 3, 2.
+  auto euler_j_wall = [&](int i, int j, const double cc[5], T fx[5]) {
+    for (int e = 1; e <= 5; ++e)
+      fx[e - 1] = (cc[0] * f(i, 1, e) + cc[1] * f(i, 2, e) + cc[2] * f(i, 3, e) +
+                   cc[3] * f(i, 4, e) + cc[4] * f(i, 5, e)) * nx(i, j, 2) +
+                  (cc[0] * g(i, 1, e) + cc[1] * g(i, 2, e) + cc[2] * g(i, 3, e) +
+                   cc[3] * g(i, 4, e) + cc[4] * g(i, 5, e)) * ny(i, j, 2);
+  };
+  // predictor_7p_i/j.F
+  auto pred_i = [&](int i, int j, T pr[5]) {
+    for (int e = 1; e <= 5; ++e)
+      pr[e - 1] = -(d3 * w(i - 3, j, e)) + d2 * w(i - 2, j, e) - d1 * w(i - 1, j, e) +
+                  d1 * w(i, j, e) - d2 * w(i + 1, j, e) + d3 * w(i + 2, j, e);
+  };
+  auto pred_j = [&](int i, int j, T pr[5]) {
+    for (int e = 1; e <= 5; ++e)
+      pr[e - 1] = -(d3 * w(i, j - 3, e)) + d2 * w(i, j - 2, e) - d1 * w(i, j - 1, e) +
+                  d1 * w(i, j, e) - d2 * w(i, j + 1, e) + d3 * w(i, j + 2, e);
+  };
+  // Newtonian stress + heat flux from face gradients (flux_visqueux_o*.F tail)
+  auto stress = [&](T mmu, T lambda, T uu, T vv, T ww, T ux, T uy, T vx, T vy, T wx, T wy, T tx,
+                    T ty, T fv[5], T gv[5]) {
+    fv[0] = T(0.0);
+    gv[0] = T(0.0);
+    fv[1] = TWOTHIRD * mmu * (TWO * ux - vy);
+    fv[2] = mmu * (uy + vx);
+    fv[3] = mmu * (wx);
+    fv[4] = (lambda * tx + uu * fv[1] + vv * fv[2] + ww * fv[3]);
+    gv[1] = mmu * (uy + vx);
+    gv[2] = TWOTHIRD * mmu * (-ux + TWO * vy);
+    gv[3] = mmu * (wy);
+    gv[4] = lambda * ty + uu * gv[1] + vv * gv[2] + ww * gv[3];
+  };
+  // flux_visqueux_o4_i.F / _o2_i.F (i-face), contour metrics shared
+  auto visc_i = [&](int i, int j, bool o4, T fv[5], T gv[5]) {
+    double volm1 = volf(i, j, 1);
+    double nx_N = HALF * (nx(i + 1, j, 1) + nx(i, j, 1));
+    double nx_S = -(HALF * (nx(i - 1, j, 1) + nx(i, j, 1)));
+    double nx_O = HALF * (nx(i - 1, j + 1, 2) + nx(i, j + 1, 2));
+    double nx_E = -(HALF * (nx(i - 1, j, 2) + nx(i, j, 2)));
+    double ny_N = HALF * (ny(i + 1, j, 1) + ny(i, j, 1));
+    double ny_S = -(HALF * (ny(i - 1, j, 1) + ny(i, j, 1)));
+    double ny_O = HALF * (ny(i - 1, j + 1, 2) + ny(i, j + 1, 2));
+    double ny_E = -(HALF * (ny(i - 1, j, 2) + ny(i, j, 2)));
+    auto grad = [&](const F2<T>& v, T& gx, T& gy) {
+      T vN, vS, vE, vO;
+      if (o4) {
+        vN = TWENTYFOURTH * (-v(i + 1, j) + 26.0 * v(i, j) - v(i - 1, j));
+        vS = TWENTYFOURTH * (-v(i, j) + 26.0 * v(i - 1, j) - v(i - 2, j));
+        auto row = [&](int jj) {
+          return (-v(i - 2, jj) + 9.0 * v(i - 1, jj) + 9.0 * v(i, jj) - v(i + 1, jj));
+        };
+        vE = ccross * (-row(j - 2) + 7.0 * row(j - 1) + 7.0 * row(j) - row(j + 1));
+        vO = ccross * (-row(j - 1) + 7.0 * row(j) + 7.0 * row(j + 1) - row(j + 2));
+      } else {
+        vN = v(i, j);
+        vS = v(i - 1, j);
+        vE = FOURTH * (v(i, j) + v(i, j - 1) + v(i - 1, j) + v(i - 1, j - 1));
+        vO = FOURTH * (v(i, j) + v(i, j + 1) + v(i - 1, j) + v(i - 1, j + 1));
+      }
+      gx = (vN * nx_N + vS * nx_S + vO * nx_O + vE * nx_E) * volm1;
+      gy = (vN * ny_N + vS * ny_S + vO * ny_O + vE * ny_E) * volm1;
+    };
+    T ux, uy, vx, vy, wx, wy, tx, ty;
+    grad(velx, ux, uy);
+    grad(vely, vx, vy);
+    grad(velz, wx, wy);
+    grad(tloc, tx, ty);
+    T uu, vv, ww, mmu, lambda;
+    if (o4) {
+      uu = 0.0625 * (-velx(i - 2, j) + 9.0 * velx(i - 1, j) + 9.0 * velx(i, j) - velx(i + 1, j));
+      vv = 0.0625 * (-vely(i - 2, j) + 9.0 * vely(i - 1, j) + 9.0 * vely(i, j) - vely(i + 1, j));
+      ww = 0.0625 * (-velz(i - 2, j) + 9.0 * velz(i - 1, j) + 9.0 * velz(i, j) - velz(i + 1, j));
+      mmu = 0.0625 * (-mu(i - 2, j) + 9.0 * mu(i - 1, j) + 9.0 * mu(i, j) - mu(i + 1, j));
+      lambda = mmu * cpprandtl;
+    } else {
+      uu = HALF * (velx(i, j) + velx(i - 1, j));
+      vv = HALF * (vely(i, j) + vely(i - 1, j));
+      ww = HALF * (velz(i, j) + velz(i - 1, j));
+      mmu = HALF * (mu(i, j) + mu(i - 1, j));
+      lambda = HALF * (mu(i, j) + mu(i - 1, j)) * cpprandtl;
+    }
+    stress(mmu, lambda, uu, vv, ww, ux, uy, vx, vy, wx, wy, tx, ty, fv, gv);
+  };
+  auto visc_j = [&](int i, int j, bool o4, T fv[5], T gv[5]) {
+    double volm1 = volf(i, j, 2);
+    double nx_N = HALF * (nx(i + 1, j - 1, 1) + nx(i + 1, j, 1));
+    double nx_S = -(HALF * (nx(i, j - 1, 1) + nx(i, j, 1)));
+    double nx_O = HALF * (nx(i, j + 1, 2) + nx(i, j, 2));
+    double nx_E = -(HALF * (nx(i, j - 1, 2) + nx(i, j, 2)));
+    double ny_N = HALF * (ny(i + 1, j - 1, 1) + ny(i + 1, j, 1));
+    double ny_S = -(HALF * (ny(i, j - 1, 1) + ny(i, j, 1)));
+    double ny_O = HALF * (ny(i, j + 1, 2) + ny(i, j, 2));
+    double ny_E = -(HALF * (ny(i, j - 1, 2) + ny(i, j, 2)));
+    auto grad = [&](const F2<T>& v, T& gx, T& gy) {
+      T vN, vS, vE, vO;
+      if (o4) {
+        auto col = [&](int ii) {
+          return (-v(ii, j - 2) + 9.0 * v(ii, j - 1) + 9.0 * v(ii, j) - v(ii, j + 1));
+        };
+        vN = ccross * (-col(i - 1) + 7.0 * col(i) + 7.0 * col(i + 1) - col(i + 2));
+        vS = ccross * (-col(i - 2) + 7.0 * col(i - 1) + 7.0 * col(i) - col(i + 1));
+        vE = TWENTYFOURTH * (-v(i, j) + 26.0 * v(i, j - 1) - v(i, j - 2));
+        vO = TWENTYFOURTH * (-v(i, j + 1) + 26.0 * v(i, j) - v(i, j - 1));
+      } else {
+        vN = FOURTH * (v(i, j) + v(i + 1, j) + v(i, j - 1) + v(i + 1, j - 1));
+        vS = FOURTH * (v(i, j) + v(i - 1, j) + v(i, j - 1) + v(i - 1, j - 1));
+        vO = v(i, j);
+        vE = v(i, j - 1);
+      }
+      gx = (vN * nx_N + vS * nx_S + vO * nx_O + vE * nx_E) * volm1;
+      gy = (vN * ny_N + vS * ny_S + vO * ny_O + vE * ny_E) * volm1;
+    };
+    T ux, uy, vx, vy, wx, wy, tx, ty;
+    grad(velx, ux, uy);
+    grad(vely, vx, vy);
+    grad(velz, wx, wy);
+    grad(tloc, tx, ty);
+    T uu, vv, ww, mmu, lambda;
+    if (o4) {
+      uu = 0.0625 * (-velx(i, j - 2) + 9.0 * velx(i, j - 1) + 9.0 * velx(i, j) - velx(i, j + 1));
+      vv = 0.0625 * (-vely(i, j - 2) + 9.0 * vely(i, j - 1) + 9.0 * vely(i, j) - vely(i, j + 1));
+      ww = 0.0625 * (-velz(i, j - 2) + 9.0 * velz(i, j - 1) + 9.0 * velz(i, j) - velz(i, j + 1));
+      mmu = 0.0625 * (-mu(i, j - 2) + 9.0 * mu(i, j - 1) + 9.0 * mu(i, j) - mu(i, j + 1));
+      lambda = mmu * cpprandtl;
+    } else {
+      uu = HALF * (velx(i, j) + velx(i, j - 1));
+      vv = HALF * (vely(i, j) + vely(i, j - 1));
+      ww = HALF * (velz(i, j) + velz(i, j - 1));
+      mmu = HALF * (mu(i, j) + mu(i, j - 1));
+      lambda = HALF * (mu(i, j) + mu(i, j - 1)) * cpprandtl;
+    }
+    stress(mmu, lambda, uu, vv, ww, ux, uy, vx, vy, wx, wy, tx, ty, fv, gv);
+  };
+  // dissipation_ducros_i/j.F = spectralradius + ducrosfordnc + scalar diss.
+  // (il, jl) is the left cell, (i, j) the right cell; kdir the face family.
+  auto dissipation = [&](int i, int j, int il, int jl, int kdir, const T pr[5], T diss[5]) {
+    T rhomr = w(i, j, 1);
+    T ur = w(i, j, 2) / rhomr;
+    T vr = w(i, j, 3) / rhomr;
+    T c2r = gam * rgaz * tloc(i, j);
+    T rhoml = w(il, jl, 1);
+    T ul = w(il, jl, 2) / rhoml;
+    T vl = w(il, jl, 3) / rhoml;
+    T c2l = gam * rgaz * tloc(il, jl);
+    T r = f_sqrt(rhomr / rhoml);
+    T rr = ONE / (ONE + r);
+    T omrr = ONE - rr;
+    T u = ul * rr + ur * omrr;
+    T v = vl * rr + vr * omrr;
+    T c2x = c2l * rr + c2r * omrr;
+    double nxf = nx(i, j, kdir), nyf = ny(i, j, kdir);
+    double nx2 = nxf * nxf + nyf * nyf;
+    T ab = f_abs(nxf * u + nyf * v);
+    T sq = f_sqrt(c2x * nx2);
+    T rspec = ab + sq;
+    // pressure sensor along the face normal direction
+    int di = (kdir == 1), dj = (kdir == 2);
+    T ks1 = f_abs(p(il, jl) - TWO * p(i, j) + p(i + di, j + dj)) /
+            f_abs(p(il, jl) + TWO * p(i, j) + p(i + di, j + dj));
+    T ks2 = f_abs(p(il - di, jl - dj) - TWO * p(il, jl) + p(i, j)) /
+            f_abs(p(il - di, jl - dj) + TWO * p(il, jl) + p(i, j));
+    T divu = (gradu(i, j, 1) + gradv(i, j, 2));
+    T divu2 = divu * divu;
+    T vort2 = (gradv(i, j, 1) - gradu(i, j, 2)) * (gradv(i, j, 1) - gradu(i, j, 2));
+    T ducros1 = divu2 / (divu2 + vort2 + 1e-15);
+    T dxm1 = HALF * (ONE - f_tanh(2.5 + 10.0 * vol(i, j) / (f_sqrt(c2r * nx2) + 1e-15) * divu));
+    divu = (gradu(il, jl, 1) + gradv(il, jl, 2));
+    divu2 = divu * divu;
+    vort2 = (gradv(il, jl, 1) - gradu(il, jl, 2)) * (gradv(il, jl, 1) - gradu(il, jl, 2));
+    T ducros2 = divu2 / (divu2 + vort2 + 1e-15);
+    T dxm2 = HALF * (ONE - f_tanh(2.5 + 10.0 * vol(il, jl) / (f_sqrt(c2l * nx2) + 1e-15) * divu));
+    T coef = f_max(ks1, ks2) * f_max(ducros1, ducros2) * f_max(dxm1, dxm2);
+    T eps2 = k2 * coef;
+    T eps4 = f_max(T(ZERO), k4 - eps2 * 12.0);
+    for (int e = 1; e <= 5; ++e) {
+      T diff = HALF * (w(i, j, e) - w(il, jl, e));
+      diss[e - 1] = rspec * (eps2 * diff + eps4 * pr[e - 1]);
+    }
+  };
+  // fluxnumassembly_i/j.F
+  auto assemble = [&](int i, int j, int kdir, const T fx[5], const T diss[5], const T fv[5],
+                      const T gv[5]) {
+    double sc1 = nx(i, j, kdir), sc2 = ny(i, j, kdir);
+    double sn = std::sqrt(sc1 * sc1 + sc2 * sc2);
+    double invsn = ONE / sn;
+    double nxloc = sc1 * invsn, nyloc = sc2 * invsn;
+    hn(i, j, 1, kdir) = fx[0] - diss[0];
+    for (int e = 2; e <= 5; ++e)
+      hn(i, j, e, kdir) = fx[e - 1] - diss[e - 1] - (fv[e - 1] * nxloc + gv[e - 1] * nyloc) * sn;
+  };
+
+  T fx[5], pr[5], fv[5], gv[5], diss[5];
+  auto iface = [&](int i, int j, bool o4) {
+    euler_i(i, j, fx);
+    pred_i(i, j, pr);
+    visc_i(i, j, o4, fv, gv);
+    dissipation(i, j, i - 1, j, 1, pr, diss);
+    assemble(i, j, 1, fx, diss, fv, gv);
+  };
+  // interior faces: j = 4 .. jm+1 (flux_num_dnc5.F90:146-159)
+  for (int j = 4; j <= jm + 1; ++j)
+    for (int i = 1; i <= im + 1; ++i) {
+      iface(i, j, true);
+      euler_j(i, j, fx);
+      pred_j(i, j, pr);
+      visc_j(i, j, true, fv, gv);
+      dissipation(i, j, i, j - 1, 2, pr, diss);
+      assemble(i, j, 2, fx, diss, fv, gv);
+    }
+  // coefnearbnd_7p.F
+  denom = 1.0 / 60.0;
+  const double c5d[5] = {-3.0 * denom, 27.0 * denom, 47.0 * denom, -13.0 * denom, 2.0 * denom};
+  const double c3d[5] = {12.0 * denom, 77.0 * denom, -43.0 * denom, 17.0 * denom, -3.0 * denom};
+  // j = 3 (flux_num_dnc5.F90:164-181)
+  for (int i = 1; i <= im + 1; ++i) {
+    const int j = 3;
+    iface(i, j, true);
+    euler_j_wall(i, j, c5d, fx);
+    pred_j(i, j, pr);
+    visc_j(i, j, true, fv, gv);
+    dissipation(i, j, i, j - 1, 2, pr, diss);
+    assemble(i, j, 2, fx, diss, fv, gv);
+  }
+  // j = 2 (flux_num_dnc5.F90:182-196)
+  for (int i = 1; i <= im + 1; ++i) {
+    const int j = 2;
+    iface(i, j, false);
+    euler_j_wall(i, j, c3d, fx);
+    pred_j(i, j, pr);
+    visc_j(i, j, false, fv, gv);
+    dissipation(i, j, i, j - 1, 2, pr, diss);
+    assemble(i, j, 2, fx, diss, fv, gv);
+  }
+  // j = 1: i-face (o2) + adiabatic no-slip wall j-face (fluxwall.F:3-50)
+  const double ct0 = 1.125, ct1 = -0.125;
+  for (int i = 1; i <= im + 1; ++i) {
+    const int j = 1;
+    iface(i, j, false);
+    T pw = ct0 * p(i, j) + ct1 * p(i, j + 1);
+    T fxrov2 = pw;
+    T mmu = mu(i, j);
+    double n2x = nx(i, j, 2), n2y = ny(i, j, 2), vf = volf(i, j, 2);
+    T ux = TWO * velx(i, j) * n2x * vf;
+    T vx = TWO * vely(i, j) * n2x * vf;
+    T wx = TWO * velz(i, j) * n2x * vf;
+    T uy = TWO * velx(i, j) * n2y * vf;
+    T vy = TWO * vely(i, j) * n2y * vf;
+    T wy = TWO * velz(i, j) * n2y * vf;
+    T fvrou2 = TWOTHIRD * mmu * (TWO * ux - vy);
+    T fvrov2 = mmu * (uy + vx);
+    T fvrow2 = mmu * (wx);
+    T gvrou2 = mmu * (uy + vx);
+    T gvrov2 = TWOTHIRD * mmu * (-ux + TWO * vy);
+    T gvrow2 = mmu * (wy);
+    hn(i, j, 1, 2) = T(ZERO);
+    hn(i, j, 2, 2) = fxrov2 * n2x - (fvrou2 * n2x + gvrou2 * n2y);
+    hn(i, j, 3, 2) = fxrov2 * n2y - (fvrov2 * n2x + gvrov2 * n2y);
+    hn(i, j, 4, 2) = -(fvrow2 * n2x + gvrow2 * n2y);
+    hn(i, j, 5, 2) = T(ZERO);
+  }
+  // balanceloop.F
+  for (int e = 1; e <= 5; ++e)
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i)
+        residu(i, j, e) = -(hn(i + 1, j, e, 1) - hn(i, j, e, 1)) - (hn(i, j + 1, e, 2) - hn(i, j, e, 2));
+}
+
+// =============================================================================
+//  Boundary conditions (srcfv/borders/*.F90, expanded in srcfv/prepro/)
+// =============================================================================
+struct Box {
+  int imin, jmin, imax, jmax;
+};
+// interface decoding common to all BCs (srcfv/borders/init_2d.F:1-44)
+struct Side {
+  int kdir, i0, j0, high, lmax;
+};
+static Side decode(int loc, const Box& b) {
+  Side s{0, 0, 0, 0, 0};
+  if (loc == 0) { s.kdir = 1; s.i0 = 1; s.lmax = b.jmax - b.jmin + 1; }
+  else if (loc == 1) { s.kdir = 1; s.i0 = -1; s.lmax = b.jmax - b.jmin + 1; s.high = 1; }
+  else if (loc == 2) { s.kdir = 2; s.j0 = 1; s.lmax = b.imax - b.imin + 1; }
+  else { s.kdir = 2; s.j0 = -1; s.lmax = b.imax - b.imin + 1; s.high = 1; }
+  return s;
+}
+
+// bc_supandsubinlet_2d (srcfv/borders/bc_supandsubinlet.F90:2-171)
+template <class T>
+void bc_inlet(T* w_p, int loc, Box b, const double* field_p, const double* nx_p,
+              const double* ny_p, double gam, int im, int jm, int lm, int gh) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<T> w{w_p, lo, ni, lo, nj, 1};
+  F3<const double> field{field_p, 1, lm, 1, gh, 1};
+  F3<const double> nx{nx_p, lo, ni + 1, lo, nj + 1, 1};
+  F3<const double> ny{ny_p, lo, ni + 1, lo, nj + 1, 1};
+  Side s = decode(loc, b);
+  const double HALF = 0.5, ONE = 1.0;
+  const double gam1 = gam - ONE;
+  for (int l = 1; l <= s.lmax; ++l) {
+    int i = b.imin + (l - 1) * s.j0 * s.j0;
+    int j = b.jmin + (l - 1) * s.i0 * s.i0;
+    T wr1 = w(i, j, 1);
+    T rom1 = ONE / wr1;
+    T wr2 = w(i, j, 2) * rom1;
+    T wr3 = w(i, j, 3) * rom1;
+    T wr4 = w(i, j, 4) * rom1;
+    T pr = gam1 * (w(i, j, 5) - wr1 * (wr2 * wr2 + wr3 * wr3 + wr4 * wr4));  // quirk: no 1/2
+    T hr = (w(i, j, 5) + pr) * rom1;
+    double wl1 = field(j, i, 1);
+    double rosm1 = ONE / wl1;
+    double wl2 = field(j, i, 2) * rosm1;
+    double wl3 = field(j, i, 3) * rosm1;
+    double wl4 = field(j, i, 4) * rosm1;
+    double pl = gam1 * (field(j, i, 5) - wl1 * (wl2 * wl2 + wl3 * wl3 + wl4 * wl4));
+    double hl = (field(j, i, 5) + pl) * rosm1;
+    T r = f_sqrt(wr1 / wl1);
+    T rr = f_sqrt(wr1 * wl1);
+    T oneonrplusone = ONE / (r + ONE);
+    T uu = (wr2 * r + wl2) * oneonrplusone;
+    T vv = (wr3 * r + wl3) * oneonrplusone;
+    T ww = (wr4 * r + wl4) * oneonrplusone;
+    T hh = (hr * r + hl) * oneonrplusone;
+    T velo2 = uu * uu + vv * vv + ww;  // quirk: ww not squared
+    T ee = HALF * velo2;
+    T sound2 = gam1 * (hh - ee);
+    T mach = f_sqrt(velo2 / sound2);
+    double nxnorm = 0, nynorm = 0, epsm = 0, eps0 = 0, epsp = 0;
+    const bool sub = val(mach) < ONE;
+    if (sub) {
+      double nxloc = nx(i, j, s.kdir), nyloc = ny(i, j, s.kdir);
+      double nsum = ONE / std::sqrt(nxloc * nxloc + nyloc * nyloc);
+      nxnorm = nxloc * nsum;
+      nynorm = nyloc * nsum;
+      T roc0 = rr * f_sqrt(sound2);
+      T rovn0 = rr * (uu * nxnorm + vv * nynorm);
+      epsm = HALF + f_sign(HALF, val(roc0 - rovn0));
+      eps0 = HALF + f_sign(HALF, val(-rovn0));
+      epsp = HALF + f_sign(HALF, val(-roc0 - rovn0));
+    }
+    for (int de = 1; de <= gh; ++de) {
+      const int ig = i - de * s.i0, jg = j - de * s.j0;
+      for (int e = 1; e <= 5; ++e) w(ig, jg, e) = T(field(l, de, e));
+      const int da = de - 1;
+      if (sub) {
+        T rod = w(ig, jg, 1);
+        T rm1 = ONE / rod;
+        T ud = w(ig, jg, 2) * rm1;
+        T vd = w(ig, jg, 3) * rm1;
+        T wd = w(ig, jg, 4) * rm1;
+        T pd = gam1 * (w(ig, jg, 5) - rod * HALF * (ud * ud + vd * vd + wd * wd));
+        T vnd = (ud * nxnorm + vd * nynorm);
+        T utd = ud - vnd * nxnorm;
+        T vtd = vd - vnd * nynorm;
+        T hl2 = (w(ig, jg, 5) + pd) * rm1;
+        const int is = i - da * s.i0, js = j - da * s.j0;
+        T ros = w(is, js, 1);
+        T rosm = ONE / ros;
+        T us = w(is, js, 2) * rosm;
+        T vs = w(is, js, 3) * rosm;
+        T ws = w(is, js, 4) * rosm;
+        T ps = gam1 * (w(is, js, 5) - ros * HALF * (us * us + vs * vs + ws * ws));
+        T vns = (us * nxnorm + vs * nynorm);
+        T uts = us - vns * nxnorm;
+        T vts = vs - vns * nynorm;
+        T hr2 = (w(is, js, 5) + ps) * rosm;
+        T r2 = f_sqrt(ros * rm1);
+        T rr2 = f_sqrt(ros * rod);
+        T oo = ONE / (r2 + ONE);
+        T u2 = (us * r2 + ud) * oo;
+        T v2 = (vs * r2 + vd) * oo;
+        T w2 = (ws * r2 + wd) * oo;
+        T ee2 = HALF * (u2 * u2 + v2 * v2 + w2 * w2);
+        T hh2 = (hr2 * r2 + hl2) * oo;
+        T snd2 = gam1 * (hh2 - ee2);
+        T roc0 = rr2 * f_sqrt(snd2);
+        T roc0m1 = ONE / roc0;
+        T ut = eps0 * uts + (ONE - eps0) * utd;
+        T vt = eps0 * vts + (ONE - eps0) * vtd;
+        T am = epsm * (ps - roc0 * vns) + (ONE - epsm) * (pd - roc0 * vnd);
+        T ap = epsp * (ps + roc0 * vns) + (ONE - epsp) * (pd + roc0 * vnd);
+        T vn = (ap - am) * HALF * roc0m1;
+        T pp = (ap + am) * HALF;
+        T bs = (pp - ps) * rr2 * rr2 * roc0m1 * roc0m1 + ros;
+        T b0 = (pp - pd) * rr2 * rr2 * roc0m1 * roc0m1 + rod;
+        T ro = eps0 * bs + (ONE - eps0) * b0;
+        T roe = pp / gam1;
+        T rou = ro * (ut + vn * nxnorm);
+        T rov = ro * (vt + vn * nynorm);
+        T row = T(field(j, de, 4));  // quirk: field(j, de, 4)
+        w(ig, jg, 1) = ro;
+        w(ig, jg, 2) = rou;
+        w(ig, jg, 3) = rov;
+        w(ig, jg, 4) = row;
+        w(ig, jg, 5) = roe + HALF * (rou * rou + rov * rov + row * row) / ro;
+      }
+    }
+  }
+}
+
+// bc_no_reflexion_2d (srcfv/borders/bc_no_reflexion.F90:8-152)
+template <class T>
+void bc_noref(T* w_p, const double* wbd_p, int loc, Box b, const double* nx_p,
+              const double* ny_p, double gam, int gh, int im, int jm, int lm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<T> w{w_p, lo, ni, lo, nj, 1};
+  F2<const double> wbd{wbd_p, 1, lm, 1};
+  F3<const double> nx{nx_p, lo, ni + 1, lo, nj + 1, 1};
+  F3<const double> ny{ny_p, lo, ni + 1, lo, nj + 1, 1};
+  Side s = decode(loc, b);
+  const double ONE = 1.0, HALF = 0.5, TWO = 2.0;
+  const double gam1 = gam - ONE;
+  const double sens = (double)(float)(s.i0 + s.j0);
+  const int i1 = s.i0 * s.i0, j1 = s.j0 * s.j0;
+  for (int l = 1; l <= s.lmax; ++l) {
+    int i = b.imin + (l - 1) * j1;
+    int j = b.jmin + (l - 1) * i1;
+    double nxloc = nx(i + s.high * i1, j + s.high * j1, s.kdir);
+    double nyloc = ny(i + s.high * i1, j + s.high * j1, s.kdir);
+    double nsum = std::sqrt(nxloc * nxloc + nyloc * nyloc);
+    double nsumi = ONE / nsum;
+    double nxnorm = nxloc * nsumi * sens;
+    double nynorm = nyloc * nsumi * sens;
+    T ro0 = w(i, j, 1);
+    T ro0m1 = ONE / ro0;
+    T uu = w(i, j, 2) * ro0m1;
+    T vv = w(i, j, 3) * ro0m1;
+    T ww = w(i, j, 4) * ro0m1;
+    T roe1 = w(i, j, 5) - HALF * ro0 * (uu * uu + vv * vv + ww * ww);
+    T p0 = gam1 * (roe1);
+    T roc0 = ro0 * f_sqrt(gam * p0 * ro0m1);
+    T roc0m1 = ONE / roc0;
+    T rovn0 = (w(i, j, 2) * nxnorm + w(i, j, 3) * nynorm);
+    double epsm = HALF + f_sign(HALF, val(roc0 - rovn0));
+    double eps0 = HALF + f_sign(HALF, val(-rovn0));
+    double epsp = HALF + f_sign(HALF, val(-roc0 - rovn0));
+    double rod = wbd(l, 1);
+    double rodm1 = ONE / rod;
+    double ud = wbd(l, 2) * rodm1;
+    double vd = wbd(l, 3) * rodm1;
+    double wdi = wbd(l, 4) * rodm1;
+    double pd = gam1 * (wbd(l, 5) - rod * HALF * (ud * ud + vd * vd + wdi * wdi));
+    double vnd = (ud * nxnorm + vd * nynorm);
+    double utd = ud - vnd * nxnorm;
+    double vtd = vd - vnd * nynorm;
+    T ros = w(i, j, 1);
+    T rosm1 = ONE / ros;
+    T us = w(i, j, 2) * rosm1;
+    T vs = w(i, j, 3) * rosm1;
+    T ws = w(i, j, 4) * rosm1;
+    T ps = gam1 * (w(i, j, 5) - ros * HALF * (us * us + vs * vs + ws * ws));
+    T vns = (us * nxnorm + vs * nynorm);
+    T uts = us - vns * nxnorm;
+    T vts = vs - vns * nynorm;
+    T ut = eps0 * uts + (ONE - eps0) * utd;
+    T vt = eps0 * vts + (ONE - eps0) * vtd;
+    T wt = eps0 * ws + (ONE - eps0) * wdi;
+    T am = epsm * (ps - roc0 * vns) + (ONE - epsm) * (pd - roc0 * vnd);
+    T ap = epsp * (ps + roc0 * vns) + (ONE - epsp) * (pd + roc0 * vnd);
+    T vn = (ap - am) * HALF * roc0m1;
+    T p = (ap + am) * HALF;
+    T bs = (p - ps) * ro0 * ro0 * roc0m1 * roc0m1 + ros;
+    T b0 = (p - pd) * ro0 * ro0 * roc0m1 * roc0m1 + rod;
+    T ro = eps0 * bs + (ONE - eps0) * b0;
+    T rom1 = ONE / ro;
+    T roe = p / gam1;
+    T rou = ro * (ut + vn * nxnorm);
+    T rov = ro * (vt + vn * nynorm);
+    T row = ro * wt;
+    T roe2 = roe + HALF * rom1 * (rou * rou + rov * rov + row * row);
+    for (int de = 1; de <= gh; ++de) {
+      const int da = de - 1;
+      const int ig = i - de * s.i0, jg = j - de * s.j0;
+      const int ia = i - da * s.i0, ja = j - da * s.j0;
+      w(ig, jg, 1) = ro;
+      w(ig, jg, 2) = rou;
+      w(ig, jg, 3) = rov;
+      w(ig, jg, 4) = row;
+      w(ig, jg, 5) = roe2;
+      ro = TWO * ro - w(ia, ja, 1);
+      rou = TWO * rou - w(ia, ja, 2);
+      rov = TWO * rov - w(ia, ja, 3);
+      row = TWO * row - w(ia, ja, 4);
+      roe2 = TWO * roe2 - w(ia, ja, 5);
+    }
+  }
+}
+
+// bc_extrapolate_o2_2d (srcfv/borders/bc_extrapolate.F90:39-71)
+template <class T>
+void bc_extrap2(T* w_p, int loc, Box b, int im, int jm, int gh, int em) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<T> w{w_p, lo, ni, lo, nj, 1};
+  Side s = decode(loc, b);
+  for (int l = 1; l <= s.lmax; ++l) {
+    int i = b.imin + (l - 1) * s.j0 * s.j0;
+    int j = b.jmin + (l - 1) * s.i0 * s.i0;
+    for (int de = 1; de <= gh; ++de) {
+      int dd1 = de - 1, dd2 = de - 2;
+      for (int e = 1; e <= em; ++e)
+        w(i - s.i0 * de, j - s.j0 * de, e) =
+            2.0 * w(i - s.i0 * dd1, j - s.j0 * dd1, e) - w(i - s.i0 * dd2, j - s.j0 * dd2, e);
+    }
+  }
+}
+
+// bc_wall_viscous_adia_2d (srcfv/borders/bc_wall_viscous.F90:2-99)
+template <class T>
+void bc_wall(T* w_p, int loc, double gam, Box b, int gh, int im, int jm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<T> w{w_p, lo, ni, lo, nj, 1};
+  Side s = decode(loc, b);
+  const double gam1 = gam - 1.0, gami = 1.0 / gam;
+  const double ct0 = 1.125, ct1 = -0.125, THIRD = 1.0 / 3.0, FOUR = 4.0, HALF = 0.5, ONE = 1.0;
+  for (int l = 1; l <= s.lmax; ++l) {
+    int i = b.imin + (l - 1) * s.j0 * s.j0;
+    int j = b.jmin + (l - 1) * s.i0 * s.i0;
+    T roe = w(i, j, 1);
+    T roem1 = ONE / roe;
+    T ue = w(i, j, 2) * roem1;
+    T ve = w(i, j, 3) * roem1;
+    T we = w(i, j, 4) * roem1;
+    T ve2 = ue * ue + ve * ve;
+    T pe = gam1 * (w(i, j, 5) - HALF * roe * ve2);
+    T roe1 = w(i + s.i0, j + s.j0, 1);
+    T roe1m1 = ONE / roe1;
+    T ue1 = w(i + s.i0, j + s.j0, 2) * roe1m1;
+    T ve1 = w(i + s.i0, j + s.j0, 3) * roe1m1;
+    T ve21 = ue1 * ue1 + ve1 * ve1;
+    T pe1 = gam1 * (w(i + s.i0, j + s.j0, 5) - HALF * roe1 * ve21);
+    T pw = ct0 * pe + ct1 * pe1;
+    T pi = THIRD * (FOUR * pw - pe);
+    T roi = f_pow(pi / pe * f_pow(roe, gam), gami);
+    T ui = -ue, vi = -ve, wi = -we;
+    T roiei = pi / gam1 + HALF * roi * (ui * ui + vi * vi);
+    for (int de = 1; de <= gh; ++de) {
+      const int ig = i - de * s.i0, jg = j - de * s.j0;
+      w(ig, jg, 1) = roi;
+      w(ig, jg, 2) = roi * ui;
+      w(ig, jg, 3) = roi * vi;
+      w(ig, jg, 4) = roi * wi;
+      w(ig, jg, 5) = roiei;
+      const int ii = i + de * s.i0, jj = j + de * s.j0;
+      roi = w(ii, jj, 1);
+      roem1 = ONE / roi;
+      ue1 = w(ii, jj, 2) * roem1;
+      ve1 = w(ii, jj, 3) * roem1;
+      T we1 = w(ii, jj, 4) * roem1;
+      roiei = w(ii, jj, 5);
+      ui = -ue1;
+      vi = -ve1;
+      wi = -we1;
+    }
+  }
+}
+
+// jn_match_2d (srcfv/borders/jn_match.F90:3-66): block copy with transform
+template <class T>
+void jn_match(T* wr_p, const int* prr, int imr, int jmr, const T* wd_p, const int* prd,
+              int imd, int jmd, const int* tr, int gh, int em) {
+  // prr/prd are Fortran (2,2): p(1,1)=imin p(2,1)=imax p(1,2)=jmin p(2,2)=jmax,
+  // stored column-major as {p11, p21, p12, p22}.
+  const int nir = imr + 2 * gh, njr = jmr + 2 * gh, nid = imd + 2 * gh, njd = jmd + 2 * gh;
+  const int lo = 1 - gh;
+  F3<T> wr{wr_p, lo, nir, lo, njr, 1};
+  F3<const T> wd{wd_p, lo, nid, lo, njd, 1};
+  auto P = [](const int* p, int a, int b) { return p[(a - 1) + 2 * (b - 1)]; };
+  int istep = tr[0] >= 0 ? 1 : -1, jstep = tr[1] >= 0 ? 1 : -1;
+  int idir = std::abs(tr[0]), jdir = std::abs(tr[1]);
+  int i1, i2, j1, j2;
+  if (istep == 1) { i1 = 1; i2 = P(prd, 2, idir) - P(prd, 1, idir) + 1; }
+  else { i1 = P(prd, 2, idir) - P(prd, 1, idir) + 1; i2 = 1; }
+  if (jstep == 1) { j1 = 1; j2 = P(prd, 2, jdir) - P(prd, 1, jdir) + 1; }
+  else { j1 = P(prd, 2, jdir) - P(prd, 1, jdir) + 1; j2 = 1; }
+  int idd = 1, jdd = 1;
+  for (int j = j1; jstep > 0 ? j <= j2 : j >= j2; j += jstep) {
+    for (int i = i1; istep > 0 ? i <= i2 : i >= i2; i += istep) {
+      int ir = P(prr, 1, 1) + idd - 1;
+      int jr = P(prr, 1, 2) + jdd - 1;
+      int id = P(prd, 1, idir) + i - 1;
+      int jd = P(prd, 1, jdir) + j - 1;
+      for (int e = 1; e <= em; ++e) wr(ir, jr, e) = wd(id, jd, e);
+      ++idd;
+    }
+    idd = 1;
+    ++jdd;
+  }
+}
+
+// interleave (w, wd) <-> Dual
+void to_dual(std::vector<Dual>& d, const double* w, const double* wd, long n) {
+  d.resize(n);
+  for (long k = 0; k < n; ++k) d[k] = Dual(w[k], wd ? wd[k] : 0.0);
+}
+void from_dual(const std::vector<Dual>& d, double* w, double* wd, long n) {
+  for (long k = 0; k < n; ++k) {
+    if (w) w[k] = d[k].v;
+    if (wd) wd[k] = d[k].d;
+  }
+}
+
+}  // namespace
+
+// =============================================================================
+//  C ABI used by oracle/__init__.py (ctypes).  All arrays Fortran order.
+//  loc: 0=Ilo 1=Ihi 2=Jlo 3=Jhi.  box: {imin, jmin, imax, jmax}.
+//  phys: {cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4}.
+// =============================================================================
+extern "C" {
+
+// computegeom_2d (srcfv/geom/computegeom.F90:3-104)
+void or_computegeom_2d(double* x0_p, double* y0_p, double* nx_p, double* ny_p, double* xc_p,
+                       double* yc_p, double* vol_p, double* volf_p, int im, int jm, int gh) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F2<double> x0{x0_p, lo, ni + 1, lo}, y0{y0_p, lo, ni + 1, lo};
+  F3<double> nx{nx_p, lo, ni + 1, lo, nj + 1, 1}, ny{ny_p, lo, ni + 1, lo, nj + 1, 1};
+  F2<double> xc{xc_p, lo, ni, lo}, yc{yc_p, lo, ni, lo}, vol{vol_p, lo, ni, lo};
+  F3<double> volf{volf_p, lo, ni, lo, nj, 1};
+  const double TWO = 2.0, HALF = 0.5, FOURTH = 0.25;
+  const int ilo = 1 - gh, ihi = im + gh + 1, jlo = 1 - gh, jhi = jm + gh + 1;
+  for (int g = 1; g <= gh; ++g)
+    for (int dummy = 0; dummy < 2; ++dummy) {
+      for (int j = jlo; j <= jhi; ++j) {
+        x0(1 - g, j) = TWO * x0(2 - g, j) - x0(3 - g, j);
+        x0(im + 1 + g, j) = TWO * x0(im + g, j) - x0(im - 1 + g, j);
+      }
+      for (int i = ilo; i <= ihi; ++i) {
+        x0(i, 1 - g) = TWO * x0(i, 2 - g) - x0(i, 3 - g);
+        x0(i, jm + 1 + g) = TWO * x0(i, jm + g) - x0(i, jm - 1 + g);
+      }
+      for (int i = ilo; i <= ihi; ++i) {
+        y0(i, 1 - g) = TWO * y0(i, 2 - g) - y0(i, 3 - g);
+        y0(i, jm + 1 + g) = TWO * y0(i, jm + g) - y0(i, jm - 1 + g);
+      }
+      for (int j = jlo; j <= jhi; ++j) {
+        y0(1 - g, j) = TWO * y0(2 - g, j) - y0(3 - g, j);
+        y0(im + 1 + g, j) = TWO * y0(im + g, j) - y0(im - 1 + g, j);
+      }
+    }
+  for (int j = 1; j <= jm + 1; ++j)
+    for (int i = 1; i <= im + 1; ++i) {
+      xc(i, j) = FOURTH * (x0(i, j) + x0(i + 1, j) + x0(i, j + 1) + x0(i + 1, j + 1));
+      yc(i, j) = FOURTH * (y0(i, j) + y0(i + 1, j) + y0(i, j + 1) + y0(i + 1, j + 1));
+      double xa = x0(i, j), ya = y0(i, j), xb = x0(i + 1, j), yb = y0(i + 1, j);
+      double xc1 = x0(i, j + 1), yc1 = y0(i, j + 1), xd = x0(i + 1, j + 1), yd = y0(i + 1, j + 1);
+      double abx = xb - xa, aby = yb - ya, acx = xc1 - xa, acy = yc1 - ya;
+      double q1 = HALF * std::fabs(abx * acy - acx * aby);
+      double dcx = xc1 - xd, dcy = yc1 - yd, dbx = xb - xd, dby = yb - yd;
+      double q2 = HALF * std::fabs(dcx * dby - dbx * dcy);
+      vol(i, j) = q1 + q2;
+      nx(i, j, 1) = y0(i, j + 1) - y0(i, j);
+      ny(i, j, 1) = x0(i, j) - x0(i, j + 1);
+      nx(i, j, 2) = y0(i, j) - y0(i + 1, j);
+      ny(i, j, 2) = x0(i + 1, j) - x0(i, j);
+    }
+  for (int g = 1; g <= gh; ++g)
+    for (int dummy = 0; dummy < 2; ++dummy) {
+      for (int j = 1 - gh; j <= jm + gh; ++j) {
+        xc(1 - g, j) = TWO * xc(2 - g, j) - xc(3 - g, j);
+        xc(im + g, j) = TWO * xc(im - 1 + g, j) - xc(im - 2 + g, j);
+      }
+      for (int i = 1 - gh; i <= im + gh; ++i) {
+        xc(i, 1 - g) = TWO * xc(i, 2 - g) - xc(i, 3 - g);
+        xc(i, jm + g) = TWO * xc(i, jm - 1 + g) - xc(i, jm - 2 + g);
+      }
+      for (int j = 1 - gh; j <= jm + gh; ++j) {
+        yc(1 - g, j) = TWO * yc(2 - g, j) - yc(3 - g, j);
+        yc(im + g, j) = TWO * yc(im - 1 + g, j) - yc(im - 2 + g, j);
+      }
+      for (int i = 1 - gh; i <= im + gh; ++i) {
+        yc(i, 1 - g) = TWO * yc(i, 2 - g) - yc(i, 3 - g);
+        yc(i, jm + g) = TWO * yc(i, jm - 1 + g) - yc(i, jm - 2 + g);
+      }
+      for (int j = 1; j <= jm; ++j) {
+        vol(1 - g, j) = TWO * vol(2 - g, j) - vol(3 - g, j);
+        vol(im + g, j) = TWO * vol(im - 1 + g, j) - vol(im - 2 + g, j);
+      }
+      for (int i = 1 - gh; i <= im + gh; ++i) {
+        vol(i, 1 - g) = TWO * vol(i, 2 - g) - vol(i, 3 - g);
+        vol(i, jm + g) = TWO * vol(i, jm - 1 + g) - vol(i, jm - 2 + g);
+      }
+      for (int k = 1; k <= 2; ++k)
+        for (int j = jlo; j <= jhi; ++j) {
+          nx(1 - g, j, k) = TWO * nx(2 - g, j, k) - nx(3 - g, j, k);
+          ny(1 - g, j, k) = TWO * ny(2 - g, j, k) - ny(3 - g, j, k);
+          nx(im + 1 + g, j, k) = TWO * nx(im + g, j, k) - nx(im - 1 + g, j, k);
+          ny(im + 1 + g, j, k) = TWO * ny(im + g, j, k) - ny(im - 1 + g, j, k);
+        }
+      for (int k = 1; k <= 2; ++k)
+        for (int i = ilo; i <= ihi; ++i) {
+          nx(i, 1 - g, k) = TWO * nx(i, 2 - g, k) - nx(i, 3 - g, k);
+          ny(i, 1 - g, k) = TWO * ny(i, 2 - g, k) - ny(i, 3 - g, k);
+          nx(i, 1 + jm + g, k) = TWO * nx(i, jm + g, k) - nx(i, g + jm - 1, k);
+          ny(i, 1 + jm + g, k) = TWO * ny(i, jm + g, k) - ny(i, g + jm - 1, k);
+        }
+    }
+  for (int j = 1; j <= jm + 1; ++j)
+    for (int i = 1; i <= im + 1; ++i) {
+      volf(i, j, 1) = TWO / (vol(i, j) + vol(i - 1, j));
+      volf(i, j, 2) = TWO / (vol(i, j) + vol(i, j - 1));
+    }
+}
+
+void or_flux_num_dnc5_2d(double* residu, const double* w, const double* nx, const double* ny,
+                         const double* vol, const double* volf, int gh, const double* phys,
+                         int im, int jm) {
+  Phys ph{phys[0], phys[1], phys[2], phys[3], phys[4], phys[5],
+          phys[6], phys[7], phys[8], phys[9], phys[10]};
+  residual<double>(residu, w, nx, ny, vol, volf, gh, ph, im, jm);
+}
+
+// flux_num_dnc5_2d_d: writes residu and residud on the interior
+void or_flux_num_dnc5_2d_d(double* residu, double* residud, const double* w, const double* wd,
+                           const double* nx, const double* ny, const double* vol,
+                           const double* volf, int gh, const double* phys, int im, int jm) {
+  Phys ph{phys[0], phys[1], phys[2], phys[3], phys[4], phys[5],
+          phys[6], phys[7], phys[8], phys[9], phys[10]};
+  const long n = (long)(im + 2 * gh) * (jm + 2 * gh) * 5;
+  std::vector<Dual> wdual, rdual;
+  to_dual(wdual, w, wd, n);
+  to_dual(rdual, residu, residud, n);
+  residual<Dual>(rdual.data(), wdual.data(), nx, ny, vol, volf, gh, ph, im, jm);
+  from_dual(rdual, residu, residud, n);
+}
+
+void or_bc_inlet(double* w, double* wd, int loc, const int* box, const double* field,
+                 const double* nx, const double* ny, double gam, int im, int jm, int lm, int gh) {
+  Box b{box[0], box[1], box[2], box[3]};
+  if (!wd) { bc_inlet<double>(w, loc, b, field, nx, ny, gam, im, jm, lm, gh); return; }
+  const long n = (long)(im + 2 * gh) * (jm + 2 * gh) * 5;
+  std::vector<Dual> d;
+  to_dual(d, w, wd, n);
+  bc_inlet<Dual>(d.data(), loc, b, field, nx, ny, gam, im, jm, lm, gh);
+  from_dual(d, w, wd, n);
+}
+void or_bc_noref(double* w, double* wd, const double* wbd, int loc, const int* box,
+                 const double* nx, const double* ny, double gam, int gh, int im, int jm, int lm) {
+  Box b{box[0], box[1], box[2], box[3]};
+  if (!wd) { bc_noref<double>(w, wbd, loc, b, nx, ny, gam, gh, im, jm, lm); return; }
+  const long n = (long)(im + 2 * gh) * (jm + 2 * gh) * 5;
+  std::vector<Dual> d;
+  to_dual(d, w, wd, n);
+  bc_noref<Dual>(d.data(), wbd, loc, b, nx, ny, gam, gh, im, jm, lm);
+  from_dual(d, w, wd, n);
+}
+void or_bc_extrap2(double* w, double* wd, int loc, const int* box, int im, int jm, int gh, int em) {
+  Box b{box[0], box[1], box[2], box[3]};
+  bc_extrap2<double>(w, loc, b, im, jm, gh, em);
+  if (wd) bc_extrap2<double>(wd, loc, b, im, jm, gh, em);
+}
+void or_bc_wall(double* w, double* wd, int loc, double gam, const int* box, int gh, int im, int jm) {
+  Box b{box[0], box[1], box[2], box[3]};
+  if (!wd) { bc_wall<double>(w, loc, gam, b, gh, im, jm); return; }
+  const long n = (long)(im + 2 * gh) * (jm + 2 * gh) * 5;
+  std::vector<Dual> d;
+  to_dual(d, w, wd, n);
+  bc_wall<Dual>(d.data(), loc, gam, b, gh, im, jm);
+  from_dual(d, w, wd, n);
+}
+void or_jn_match(double* wr, const int* prr, int imr, int jmr, const double* wd, const int* prd,
+                 int imd, int jmd, const int* tr, int gh, int em) {
+  jn_match<double>(wr, prr, imr, jmr, wd, prd, imd, jmd, tr, gh, em);
+}
+
+// compute_norml2inf / compute_norml2 (srcfv/norm.F90:2-77)
+void or_compute_norml2inf(double* norm, double* ninf, const double* rhs, int im, int jm, int gh) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> r{rhs, lo, ni, lo, nj, 1};
+  double l2[5] = {0, 0, 0, 0, 0}, li[5] = {0, 0, 0, 0, 0};
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i)
+      for (int e = 1; e <= 5; ++e) {
+        l2[e - 1] = l2[e - 1] + r(i, j, e) * r(i, j, e);
+        li[e - 1] = li[e - 1] + std::pow(r(i, j, e), 10.0);
+      }
+  for (int e = 0; e < 5; ++e) {
+    norm[e] = std::sqrt(l2[e]);
+    ninf[e] = std::pow(li[e], 0.1);
+  }
+}
+void or_compute_norml2(double* norm, double* nmoy, const double* rhs, int im, int jm, int gh) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> r{rhs, lo, ni, lo, nj, 1};
+  double l2[5] = {0, 0, 0, 0, 0};
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i)
+      for (int e = 1; e <= 5; ++e) l2[e - 1] = l2[e - 1] + r(i, j, e) * r(i, j, e);
+  for (int e = 0; e < 5; ++e) {
+    norm[e] = std::sqrt(l2[e]);
+    nmoy[e] = norm[e] / (im * jm);
+  }
+}
+
+// testvector (misc/ComputeJacobian.f90:357-374)
+void or_testvector(double* wd, int m, int l, int k, int gh, int im, int jm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  std::memset(wd, 0, sizeof(double) * (size_t)ni * nj * 5);
+  F3<double> W{wd, lo, ni, lo, nj, 1};
+  const int step = 1 + 2 * gh;
+  for (int j = k + 1; j <= jm; j += step)
+    for (int i = l + 1; i <= im; i += step) W(i, j, m + 1) = 1.0;
+}
+
+// computejacobianfromjv[_relaxed][_withjn] (misc/ComputeJacobian.f90:292-355,
+// 503-570, 847-926 and the non-relaxed join variant that follows it).
+// coefdiag == nullptr selects the plain variant; withjn selects the i-periodic
+// column rule and the dummy row 5*im*jm-2.
+void or_computejacobianfromjv(double* jac, int* ia, int* ja, const double* resd, int m, int l,
+                              int k, int gh, int im, int jm, const double* coefdiag, int withjn) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> rd{resd, lo, ni, lo, nj, 1};
+  const int step = 1 + 2 * gh;
+  const int dummy = withjn ? 5 * im * jm - 2 : 5 * im * jm - 1;
+  for (int e = 1; e <= 5; ++e)
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i) {
+        long cur = (long)(i - 1) + (long)(j - 1) * im + (long)(e - 1) * im * jm +
+                   (long)k * im * jm * 5 + (long)l * im * jm * 5 * step +
+                   (long)m * im * jm * 5 * step * step;
+        ia[cur] = e - 1 + (j - 1) * 5 + (i - 1) * jm * 5;
+        int valj = (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
+        if (valj >= jm) { ia[cur] = dummy; ja[cur] = 0; jac[cur] = 0.0; continue; }
+        int vali;
+        if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
+        else if (i <= l - gh) vali = withjn ? im - 1 - 2 * gh + l : im + 1;
+        else vali = (i - (l + 1) + gh) / step * step + l;
+        if (vali >= im) {
+          if (withjn && vali - im <= gh - 1) vali = l;
+          else { ia[cur] = dummy; ja[cur] = 0; jac[cur] = 0.0; continue; }
+        }
+        ja[cur] = m + valj * 5 + vali * jm * 5;
+        double v = -rd(i, j, e);
+        if (coefdiag && ia[cur] == ja[cur]) v = coefdiag[(i - 1) + (long)im * (j - 1)] - rd(i, j, e);
+        jac[cur] = v;
+      }
+}
+
+// set_bndbl_2d (set_bnd.f90:2-24)
+void or_set_bndbl_2d(const double* w, double* field, double* wbd, int im, int jm, int gh) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> W{w, lo, ni, lo, nj, 1};
+  F3<double> fld{field, 1, jm, 1, gh, 1};
+  F2<double> bd{wbd, 1, im + gh, 1};
+  for (int j = 1; j <= jm; ++j)
+    for (int d = 1; d <= gh; ++d)
+      for (int e = 1; e <= 5; ++e) fld(j, d, e) = W(1 - d, j, e);
+  for (int i = 1; i <= im + gh; ++i)
+    for (int e = 1; e <= 5; ++e) bd(i, e) = W(i - gh, jm, e);
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""ORACLE -- test infrastructure only.
+
+Driver-level sequences of the reference, expressed over either checker
+(``oracle.Oracle`` or ``oracle.Reference``): geometry set-up, boundary-condition
+order, residual, tangent JVP and the colored-seed Jacobian assembly.
+
+Follows BROADCAST_npz.py:702 (geometry), :1018-1031 (BC order + residual),
+:1068-1127 (245-seed Jacobian loop) for the flat-plate block, and
+cylinder.py:496-527 + handleBC.py:129-242 (join, BC order) for the O-mesh.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from broadcast_amd.cases import Case, join_boxes
+
+
+def geometry(lib, case: Case):
+    im, jm, gh = case.im, case.jm, case.gh
+    ni, nj = case.ni, case.nj
+    g = dict(
+        x0=np.array(case.x0, order="F", copy=True),
+        y0=np.array(case.y0, order="F", copy=True),
+        nx=np.zeros((ni + 1, nj + 1, 2), order="F"),
+        ny=np.zeros((ni + 1, nj + 1, 2), order="F"),
+        xc=np.zeros((ni, nj), order="F"),
+        yc=np.zeros((ni, nj), order="F"),
+        vol=np.zeros((ni, nj), order="F"),
+        volf=np.zeros((ni, nj, 2), order="F"),
+    )
+    lib.computegeom_2d(g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"],
+                       g["volf"], im, jm, gh)
+    if case.kind == "cyl":
+        # cylinder.py:499-527: periodic ghost metrics replace the linear
+        # extrapolation of computegeom_2d.
+        (prr1, prd2, tr), (prr2, prd1, _) = join_boxes(im, jm, gh)
+        for key in ("xc", "yc", "vol"):
+            a = g[key][:, :, None].copy(order="F")
+            lib.jn_match_2d(a, prr1, gh, gh, gh, gh, im, jm, a, prd2, gh, gh, gh, gh, im, jm, tr)
+            lib.jn_match_2d(a, prr2, gh, gh, gh, gh, im, jm, a, prd1, gh, gh, gh, gh, im, jm, tr)
+            g[key][:, :] = a[:, :, 0]
+        jmin, jmax = 1 - gh, jm + gh
+
+        def box(a, b, c, d):
+            return np.asfortranarray(np.array([[a, b], [c, d]], dtype=np.int32))
+
+        prr1g = box(im + 2, jmin, im + 1 + gh, jmax + 1)
+        prd1g = box(im + 1 - gh, jmin, im, jmax + 1)
+        prr2g = box(1 - gh, jmin, 0, jmax + 1)
+        prd2g = box(2, jmin, 1 + gh, jmax + 1)
+        for key in ("nx", "ny"):
+            a = g[key]
+            lib.jn_match_2d(a, prr1g, gh, gh, gh, gh, im + 1, jm + 1, a, prd2g, gh, gh, gh, gh,
+                            im + 1, jm + 1, tr)
+            lib.jn_match_2d(a, prr2g, gh, gh, gh, gh, im + 1, jm + 1, a, prd1g, gh, gh, gh, gh,
+                            im + 1, jm + 1, tr)
+        vol, volf = g["vol"], g["volf"]
+        for j in range(gh, jm + 1 + gh):
+            for i in range(gh, im + 1 + gh):
+                volf[i, j, 0] = 2.0 / (vol[i, j] + vol[i - 1, j])
+                volf[i, j, 1] = 2.0 / (vol[i, j] + vol[i, j - 1])
+    return g
+
+
+def apply_bcs(lib, case: Case, g, w, wd=None):
+    """Ghost filling in the reference's order.  With wd given, each linearised
+    BC is applied to (w, wd) before the primal one, as handleBC.applyBC does
+    (handleBC.py:184-240); for the flat plate the primal BCs were already
+    applied, so the tangent call alone is what BROADCAST_npz.py:1079-1082 runs."""
+    im, jm, gh = case.im, case.jm, case.gh
+    gam = case.phys.gam
+    it = case.interf
+    if case.kind == "bl":
+        lib.bc_supandsubinlet_2d(w, "Ilo", it["Ilo"], case.field, g["nx"], g["ny"], gam, im, jm,
+                                 gh, wd=wd)
+        lib.bc_no_reflexion_2d(w, case.wbd, "Jhi", it["Jhi"], g["nx"], g["ny"], gam, gh, im, jm,
+                               wd=wd)
+        lib.bc_extrapolate_o2_2d(w, "Ihi", it["Ihi"], im, jm, gh, wd=wd)
+        lib.bc_wall_viscous_adia_2d(w, "Jlo", gam, it["Jlo"], gh, im, jm, wd=wd)
+    else:
+        lib.bc_no_reflexion_2d(w, case.wbd, "Jhi", it["Jhi"], g["nx"], g["ny"], gam, gh, im, jm,
+                               wd=wd)
+        lib.bc_wall_viscous_adia_2d(w, "Jlo", gam, it["Jlo"], gh, im, jm, wd=wd)
+        (prr1, prd2, tr), (prr2, prd1, _) = join_boxes(im, jm, gh)
+        for _ in range(2):  # interf3 and interf4 both apply the two joins
+            for arr in ([wd] if wd is not None else []) + [w]:
+                lib.jn_match_2d(arr, prr1, gh, gh, gh, gh, im, jm, arr, prd2, gh, gh, gh, gh, im,
+                                jm, tr)
+                lib.jn_match_2d(arr, prr2, gh, gh, gh, gh, im, jm, arr, prd1, gh, gh, gh, gh, im,
+                                jm, tr)
+
+
+def residual(lib, case: Case, g, w):
+    res = np.zeros_like(w, order="F")
+    lib.flux_num_dnc5_2d(res, w, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"],
+                         g["vol"], g["volf"], case.gh, *case.phys.scheme_args(), case.im,
+                         case.jm)
+    return res
+
+
+def jvp(lib, case: Case, g, w, wd):
+    """dR/dw . wd for interior wd: linearised BCs fill wd's ghosts, then the
+    tangent residual (BROADCAST_npz.py:1079-1085)."""
+    w = np.array(w, order="F", copy=True)
+    wd = np.array(wd, order="F", copy=True)
+    apply_bcs(lib, case, g, w, wd=wd)
+    res = np.zeros_like(w, order="F")
+    resd = np.zeros_like(w, order="F")
+    lib.flux_num_dnc5_2d_d(res, resd, w, wd, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"],
+                           g["yc"], g["vol"], g["volf"], case.gh, *case.phys.scheme_args(),
+                           case.im, case.jm)
+    return res, resd
+
+
+def colored_jacobian(lib, case: Case, g, w, coefdiag=None, prune=2e-16):
+    """The reference's 245-seed assembly (BROADCAST_npz.py:1068-1137): one-hot
+    seeds with period 2gh+1, tangent BCs + JVP per seed, scatter into COO, drop
+    |J| <= prune, then COO -> CSR summing duplicates (misc/PETSc_func.py:71-95).
+    Returns a scipy.sparse.csr_matrix of -dR/dw (+ coefdiag on the diagonal)."""
+    import scipy.sparse as sp
+
+    im, jm, gh = case.im, case.jm, case.gh
+    step = 2 * gh + 1
+    nb = im * jm * step * step * 25
+    jac = np.zeros(nb)
+    ia = np.zeros(nb, dtype=np.int32)
+    ja = np.zeros(nb, dtype=np.int32)
+    withjn = case.kind == "cyl"
+    wd = np.zeros_like(w, order="F")
+    for m in range(5):
+        for l in range(step):
+            for k in range(step):
+                lib.testvector(wd, m, l, k, gh, im, jm)
+                _, resd = jvp(lib, case, g, w, wd)
+                lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                          coefdiag=coefdiag, withjn=withjn)
+    keep = np.abs(jac) > prune
+    n = 5 * im * jm
+    return sp.csr_matrix((jac[keep], (ia[keep], ja[keep])), shape=(n, n))
+
+
+def brute_jacobian(lib, case: Case, g, w):
+    """Column-by-column -dR/dw with unit seeds on every interior unknown
+    (5 im jm JVPs); independent of any coloring, used to pin the colored
+    assembly on tiny meshes.  Rows/cols in the C-order ravel of (im, jm, 5)."""
+    im, jm, gh = case.im, case.jm, case.gh
+    n = 5 * im * jm
+    J = np.zeros((n, n))
+    for i in range(im):
+        for j in range(jm):
+            for m in range(5):
+                wd = np.zeros_like(w, order="F")
+                wd[gh + i, gh + j, m] = 1.0
+                _, resd = jvp(lib, case, g, w, wd)
+                col = (i * jm + j) * 5 + m
+                J[:, col] = -np.ravel(resd[gh:gh + im, gh:gh + jm, :])
+    return J

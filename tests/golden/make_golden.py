@@ -1,0 +1,107 @@
+"""Writes the golden vectors of the FP64 residual / tangent / adjoint path.
+
+Every expected output here is produced by the reference Fortran itself,
+compiled from /root/reference by oracle/Makefile.ref into
+oracle/_ref/libbroadcast_ref.so and called through its Fortran ABI.  Inputs
+are the seeded synthetic cases of broadcast_amd.cases.  Run from the
+repository root:
+
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+
+Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from broadcast_amd.cases import bl_case, cylinder_case  # noqa: E402
+import oracle  # noqa: E402
+from oracle import driver as D  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def case_fixture(R, case, seed):
+    g = D.geometry(R, case)
+    w_in = np.array(case.w, order="F", copy=True)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(R, case, g, w)
+    res = D.residual(R, case, g, w)
+    norm, ninf = R.compute_norml2inf(res, case.im, case.jm, case.gh)
+    norm2, nmoy = R.compute_norml2(res, case.im, case.jm, case.gh)
+    rng = np.random.default_rng(seed)
+    gh = case.gh
+    wd = np.zeros((2,) + w.shape)
+    resd = np.zeros((2,) + w.shape)
+    wd_bc = np.zeros((2,) + w.shape)
+    for s in range(2):
+        d = np.zeros_like(w, order="F")
+        d[gh:-gh, gh:-gh, :] = rng.standard_normal((case.im, case.jm, 5))
+        wd[s] = d
+        wcopy = np.array(w, order="F", copy=True)
+        dcopy = np.array(d, order="F", copy=True)
+        D.apply_bcs(R, case, g, wcopy, wd=dcopy)
+        wd_bc[s] = dcopy
+        _, rd = D.jvp(R, case, g, w, d)
+        resd[s] = rd
+    # adjoint of the flux routine alone (no BC adjoint): wb = (dR/dw)^T resb
+    resb = np.zeros_like(w, order="F")
+    resb[gh:-gh, gh:-gh, :] = rng.standard_normal((case.im, case.jm, 5))
+    resb_in = resb.copy(order="F")
+    wb = np.zeros_like(w, order="F")
+    r2 = np.zeros_like(w, order="F")
+    R.flux_num_dnc5_2d_b(r2, resb, w, wb, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"],
+                         g["vol"], g["volf"], gh, *case.phys.scheme_args(), case.im, case.jm)
+    ph = case.phys
+    return dict(
+        kind=np.array(case.kind), im=case.im, jm=case.jm, gh=gh,
+        phys=np.array(ph.scheme_args()), mach=ph.mach, reynolds=ph.reynolds,
+        x0_in=case.x0, y0_in=case.y0, w_in=w_in,
+        field=case.field if case.field is not None else np.zeros((0,)),
+        wbd=case.wbd,
+        **{f"interf_{k}": v for k, v in case.interf.items()},
+        **{f"geo_{k}": v for k, v in g.items()},
+        w_bc=w, res=res, norm=norm, ninf=ninf, nmoy=nmoy,
+        wd=wd, wd_bc=wd_bc, resd=resd, resb=resb_in, wb=wb,
+    )
+
+
+def jac_fixture(R, case):
+    g = D.geometry(R, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(R, case, g, w)
+    rng = np.random.default_rng(77)
+    coefdiag = np.asfortranarray(rng.uniform(0.5, 1.5, (case.im, case.jm)))
+    J = D.colored_jacobian(R, case, g, w, coefdiag=coefdiag)
+    J.sort_indices()
+    return dict(kind=np.array(case.kind), im=case.im, jm=case.jm, gh=case.gh,
+                phys=np.array(case.phys.scheme_args()), x0_in=case.x0, y0_in=case.y0,
+                w_in=case.w, field=case.field if case.field is not None else np.zeros((0,)),
+                wbd=case.wbd, **{f"interf_{k}": v for k, v in case.interf.items()},
+                w_bc=w, coefdiag=coefdiag, indptr=J.indptr.astype(np.int32),
+                indices=J.indices.astype(np.int32), data=J.data)
+
+
+def main():
+    R = oracle.Reference()
+    fixtures = {
+        "bl24x16": case_fixture(R, bl_case(24, 16, seed=1), 11),
+        "bl13x9": case_fixture(R, bl_case(13, 9, seed=3, reynolds=3e5), 13),
+        "cyl28x12": case_fixture(R, cylinder_case(28, 12, seed=2), 12),
+        "jac_bl14x10": jac_fixture(R, bl_case(14, 10, seed=4)),
+        "jac_cyl14x8": jac_fixture(R, cylinder_case(14, 8, seed=5)),
+    }
+    for name, fx in fixtures.items():
+        path = os.path.join(OUT, f"{name}.npz")
+        np.savez_compressed(path, **fx)
+        print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,47 @@
+"""Loading of the golden fixtures into cases the checkers can run."""
+import os
+
+import numpy as np
+
+from broadcast_amd.cases import Case, Physics
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False))
+
+
+def case_from(fx):
+    ph = fx["phys"]
+    phys = Physics(mach=1.0, reynolds=1.0)  # scheme arguments are overridden below
+    kind = str(fx["kind"])
+    case = Case(name=kind, kind=kind, im=int(fx["im"]), jm=int(fx["jm"]), gh=int(fx["gh"]),
+                phys=phys, x0=np.asfortranarray(fx["x0_in"]), y0=np.asfortranarray(fx["y0_in"]),
+                w=np.asfortranarray(fx["w_in"]),
+                field=np.asfortranarray(fx["field"]) if fx["field"].size else None,
+                wbd=np.asfortranarray(fx["wbd"]),
+                interf={k: np.asfortranarray(fx[f"interf_{k}"]) for k in ("Ilo", "Ihi", "Jlo", "Jhi")})
+    # keep the exact scheme arguments the fixture was produced with
+    args = tuple(float(v) for v in ph)
+    case.phys = _FixedPhys(args, phys.gam if len(args) < 4 else args[3])
+    return case
+
+
+class _FixedPhys:
+    def __init__(self, args, gam):
+        self._args = args
+        self.gam = gam
+
+    def scheme_args(self):
+        return self._args
+
+
+def close(a, b, rtol, scale=None):
+    """Normwise-scaled comparison: |a-b| <= rtol * max|b| per variable (last axis)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if scale is None:
+        scale = np.max(np.abs(b).reshape(-1, b.shape[-1]), axis=0) if b.ndim >= 2 else np.max(np.abs(b))
+    err = np.max(np.abs(a - b).reshape(-1, b.shape[-1]), axis=0) if b.ndim >= 2 else np.max(np.abs(a - b))
+    return np.all(err <= rtol * np.maximum(scale, 1e-300)), err / np.maximum(scale, 1e-300)

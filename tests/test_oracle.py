@@ -1,0 +1,177 @@
+"""Pins the oracle (C++ restatement, oracle/bcoracle.cpp) before it is trusted.
+
+* against the golden vectors written by the reference Fortran itself
+  (tests/golden/make_golden.py);
+* against what the mathematics fixes: a fluid at rest gives R = 0 on any mesh
+  (every face flux reduces to p n and each closed cell sums its normals to 0),
+  the tangent equals central finite differences of the residual, the tangent is
+  linear in wd, and the colored 245-seed Jacobian equals the brute-force
+  unit-seed Jacobian;
+* against the reference library live, when it was built in this container.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import driver as D
+from broadcast_amd.cases import bl_case, cylinder_case
+from golden_util import load, case_from, close
+
+CASES = ["bl24x16", "bl13x9", "cyl28x12"]
+
+
+@pytest.fixture(scope="module")
+def O():
+    return oracle.Oracle()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_geometry_and_bcs_bit_exact(O, name):
+    fx = load(name)
+    case = case_from(fx)
+    g = D.geometry(O, case)
+    for k in ("nx", "ny", "xc", "yc", "vol", "volf"):
+        np.testing.assert_array_equal(g[k], fx[f"geo_{k}"], err_msg=k)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    np.testing.assert_array_equal(w, fx["w_bc"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_residual_vs_reference(O, name):
+    fx = load(name)
+    case = case_from(fx)
+    g = {k: fx[f"geo_{k}"] for k in ("x0", "y0", "nx", "ny", "xc", "yc", "vol", "volf")}
+    res = D.residual(O, case, g, np.asfortranarray(fx["w_bc"]))
+    ok, err = close(res, fx["res"], 1e-13)
+    assert ok, err
+    norm, ninf = O.compute_norml2inf(res, case.im, case.jm, case.gh)
+    np.testing.assert_allclose(norm, fx["norm"], rtol=1e-12)
+    np.testing.assert_allclose(ninf, fx["ninf"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_tangent_vs_reference(O, name):
+    fx = load(name)
+    case = case_from(fx)
+    g = {k: fx[f"geo_{k}"] for k in ("x0", "y0", "nx", "ny", "xc", "yc", "vol", "volf")}
+    w = np.asfortranarray(fx["w_bc"])
+    for s in range(2):
+        wcopy = np.array(w, order="F", copy=True)
+        d = np.array(fx["wd"][s], order="F", copy=True)
+        D.apply_bcs(O, case, g, wcopy, wd=d)
+        np.testing.assert_allclose(d, fx["wd_bc"][s], rtol=1e-13, atol=1e-13 * np.abs(d).max())
+        _, resd = D.jvp(O, case, g, w, fx["wd"][s])
+        ok, err = close(resd, fx["resd"][s], 1e-12)
+        assert ok, err
+
+
+def _norms_of_rest(O, case):
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    p = case.phys.p_inf
+    w[:, :, 0] = 1.0
+    w[:, :, 1:4] = 0.0
+    w[:, :, 4] = p / (case.phys.gam - 1.0)
+    wbd = np.zeros_like(case.wbd)
+    wbd[:, 0] = 1.0
+    wbd[:, 4] = p / (case.phys.gam - 1.0)
+    case.wbd = np.asfortranarray(wbd)
+    if case.field is not None:
+        f = np.zeros_like(case.field)
+        f[..., 0] = 1.0
+        f[..., 4] = p / (case.phys.gam - 1.0)
+        case.field = np.asfortranarray(f)
+    D.apply_bcs(O, case, g, w)
+    res = D.residual(O, case, g, w)
+    # face fluxes are O(p |n|); the residual of a fluid at rest is pure rounding
+    scale = p * np.abs(g["nx"]).max()
+    return np.abs(res[3:-3, 3:-3, :]).max() / scale
+
+
+# Only on the flat-plate block: on the O-mesh the reference's ghost j-face
+# normals of column im+1 come from linearly extrapolated nodes (computegeom_2d
+# then the partial periodic copy of cylinder.py:514-522), so the viscous
+# contour of the last i-face does not close and a fluid at rest keeps an O(h^2)
+# heat-flux residual there, in the reference as in the oracle.
+@pytest.mark.parametrize("mk", [lambda: bl_case(20, 12, seed=0),
+                                lambda: bl_case(9, 7, seed=0, ratio=1.2, dy1=1e-2)])
+def test_fluid_at_rest_has_zero_residual(O, mk):
+    assert _norms_of_rest(O, mk()) < 1e-13
+
+
+@pytest.mark.parametrize("mk", [lambda: bl_case(16, 10, seed=7),
+                                lambda: cylinder_case(14, 9, seed=8)])
+def test_tangent_is_derivative(O, mk):
+    case = mk()
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    rng = np.random.default_rng(3)
+    wd = np.zeros_like(w, order="F")
+    wd[3:-3, 3:-3, :] = rng.standard_normal((case.im, case.jm, 5)) * np.abs(w[3:-3, 3:-3, :])
+    _, resd = D.jvp(O, case, g, w, wd)
+    h = 1e-6
+    rp, rm = [], []
+    for sgn, out in ((1, rp), (-1, rm)):
+        ws = np.array(w + sgn * h * wd, order="F")
+        D.apply_bcs(O, case, g, ws)
+        out.append(D.residual(O, case, g, ws))
+    fd = (rp[0] - rm[0]) / (2 * h)
+    ok, err = close(resd[3:-3, 3:-3], fd[3:-3, 3:-3], 1e-6)
+    assert ok, err
+    # linearity in wd
+    _, r2 = D.jvp(O, case, g, w, 2.5 * wd)
+    ok, err = close(r2, 2.5 * resd, 1e-14)
+    assert ok, err
+
+
+def test_colored_equals_bruteforce(O):
+    case = bl_case(10, 8, seed=9)
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    Jc = D.colored_jacobian(O, case, g, w, prune=0.0).toarray()
+    Jb = D.brute_jacobian(O, case, g, w)
+    scale = np.abs(Jb).max(axis=1, keepdims=True)
+    assert np.all(np.abs(Jc - Jb) <= 1e-13 * np.maximum(scale, 1e-300))
+
+
+@pytest.mark.parametrize("name", ["jac_bl14x10", "jac_cyl14x8"])
+def test_colored_jacobian_vs_reference(O, name):
+    import scipy.sparse as sp
+
+    fx = load(name)
+    case = case_from(fx)
+    g = D.geometry(O, case)
+    w = np.asfortranarray(fx["w_bc"])
+    J = D.colored_jacobian(O, case, g, w, coefdiag=fx["coefdiag"])
+    n = 5 * case.im * case.jm
+    Jref = sp.csr_matrix((fx["data"], fx["indices"], fx["indptr"]), shape=(n, n))
+    diff = abs(J - Jref)
+    rowscale = abs(Jref).max(axis=1).toarray().ravel()
+    assert np.all(diff.max(axis=1).toarray().ravel() <= 1e-12 * rowscale)
+
+
+@pytest.mark.skipif(not os.path.exists(oracle.REF_SO), reason="reference library not built here")
+def test_scatter_indices_match_reference(O):
+    R = oracle.Reference()
+    case = bl_case(15, 9, seed=2)
+    im, jm, gh = case.im, case.jm, case.gh
+    nb = im * jm * 49 * 25
+    rng = np.random.default_rng(0)
+    resd = np.asfortranarray(rng.standard_normal((case.ni, case.nj, 5)))
+    cd = np.asfortranarray(rng.standard_normal((im, jm)))
+    for withjn in (False, True):
+        for coef in (None, cd):
+            outs = []
+            for lib in (O, R):
+                jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+                for m, l, k in [(0, 0, 0), (2, 5, 3), (4, 6, 6), (1, 3, 0), (3, 6, 2)]:
+                    lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                              coefdiag=coef, withjn=withjn)
+                outs.append((jac, ia, ja))
+            for a, b in zip(*outs):
+                np.testing.assert_array_equal(a, b)
