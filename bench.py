@@ -1,0 +1,229 @@
+#!/usr/bin/env python
+"""Benchmark of the FP64 residual + JVP path on one MI355X.
+
+Workload (one "step"): the per-Newton-iteration work of the reference driver
+on the 2-D cylinder O-mesh of 512 (azimuthal) x 384 (radial) cells
+(M = 0.1, Re_D = 50, free stream + 1e-4 seeded noise): the boundary-condition
+sequence, one residual, and the full colored Jacobian -- 245 one-hot seeds,
+each with its linearised BCs, tangent residual and scatter into the reference's
+COO layout (BROADCAST_npz.py:1018-1127 / cylinder.py:853-980).
+
+value = Mcells/s = cells per step / step time.  Inputs are resident in HBM
+before the timed region.  The dominant kernel's roofline figure comes from HIP
+events recorded around each of its launches on the stream the library uses.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "Mcells/s residual+JVP FP64 (%HBM peak); Newton time-to-1e-10 learned vs plain"
+
+# algorithmic bytes (per unit) used for the roofline figures
+RESIDUAL_B_PER_CELL = 136          # w 40 + nx,ny 32 + vol 8 + volf 16 + residu 40
+JVP_B_PER_SEED_CELL = 80           # wd in 40 + resd out 40
+JVP_SHARED_B_PER_CELL = 96         # w 40 + metrics 56, once per launch
+
+
+def kernel_ms(flow, name):
+    from broadcast_amd import _lib
+
+    tot, n = C.c_double(), C.c_long()
+    _lib.check(_lib.lib().bca_profile_read(flow.ctx.h, name.encode(), C.byref(tot), C.byref(n)),
+               "bca_profile_read")
+    return tot.value, n.value
+
+
+def cpu_baseline(case, budget_s=20.0):
+    """The oracle (the C++ restatement of the reference, g++ -O2, serial) on
+    the host: one residual plus a bounded sample of the 245 seed passes,
+    scaled to a full step.  (The flang build of the reference Fortran in
+    oracle/_ref computes the same numbers but runs its residual ~20x slower
+    than the restatement, so it would flatter the GPU; it is not used here.)"""
+    import numpy as np
+    import oracle
+    from oracle import driver as D
+
+    kind = "port"
+    lib = oracle.Oracle()
+    g = D.geometry(lib, case)
+    w = np.array(case.w, order="F", copy=True)
+    t0 = time.perf_counter()
+    D.apply_bcs(lib, case, g, w)
+    D.residual(lib, case, g, w)
+    t_res = time.perf_counter() - t0
+    im, jm, gh = case.im, case.jm, case.gh
+    nb = im * jm * 49 * 25
+    jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    wd = np.zeros_like(w, order="F")
+    seeds = [(m, l, k) for m in range(5) for l in range(7) for k in range(7)]
+    done, t_seed = 0, 0.0
+    for (m, l, k) in seeds:
+        t1 = time.perf_counter()
+        lib.testvector(wd, m, l, k, gh, im, jm)
+        _, resd = D.jvp(lib, case, g, w, wd)
+        lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm, withjn=case.kind == "cyl")
+        t_seed += time.perf_counter() - t1
+        done += 1
+        if t_res + t_seed > budget_s:
+            break
+    step_s = t_res + t_seed / done * len(seeds)
+    return {
+        "value": im * jm / step_s / 1e6,
+        "unit": "Mcells/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"1 BC+residual and {done} of 245 seed passes (testvector + tangent BCs + JVP + "
+                  f"scatter) on the {im}x{jm} O-mesh, serial, scaled to a full step "
+                  f"({t_res:.2f} s residual, {t_seed / done:.2f} s per seed)",
+        "host_cpu": _cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--im", type=int, default=512)
+    ap.add_argument("--jm", type=int, default=384)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures the single-GPU path only: every box here has one MI355X; "
+              "run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from broadcast_amd import _lib
+    from broadcast_amd.cases import cylinder_case
+    from broadcast_amd.flow import Flow, to_device
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    case = cylinder_case(args.im, args.jm, seed=1000 * 1 + 0)
+    cells = case.im * case.jm
+    f = Flow(case)
+    w = to_device(case.w)
+    res = f.new_state()
+    bufs = f.jacobian_buffers()
+
+    def step(first=False):
+        f.apply_bcs(w)
+        f.residual(w, res)
+        f.jacobian_colored(w, bufs=bufs, write_index=first)
+
+    print(f"bench: {case.name} cells={cells} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
+          flush=True)
+    step(first=True)
+    for _ in range(max(args.warmup - 1, 0)):
+        step()
+    torch.cuda.synchronize()
+
+    # residual alone (same kernels as inside the step), for its own figure
+    nres = 20
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(nres):
+        f.apply_bcs(w)
+        f.residual(w, res)
+    ev1.record()
+    torch.cuda.synchronize()
+    res_ms = ev0.elapsed_time(ev1) / nres
+    face64_ms, face64_n = kernel_ms(f, "k_face<f64>")
+
+    # timed region
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / args.steps
+    face5_ms, face5_n = kernel_ms(f, "k_face<dual5>")
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 0), "bca_profile_enable")
+    print(f"bench: {ms_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f})", file=sys.stderr, flush=True)
+
+    # dominant kernel: the 5-direction tangent face kernel, one launch per color
+    avg5 = face5_ms / max(face5_n, 1)
+    bytes5 = (5 * JVP_B_PER_SEED_CELL + JVP_SHARED_B_PER_CELL) * cells
+    ach5 = bytes5 / (avg5 * 1e-3) / 1e9
+    avg64 = face64_ms / max(face64_n, 1)
+    ach_res = RESIDUAL_B_PER_CELL * cells / (res_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": cells / (ms_step * 1e-3) / 1e6,
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cylinder O-mesh {case.im}x{case.jm} (M=0.1, Re_D=50, seeded free stream): "
+                        "BCs + residual + full 245-seed colored Jacobian into COO values",
+            "cells": cells,
+            "seeds": 245,
+            "jacobian_entries": f.nbentry,
+        },
+        "roofline": {
+            "kernel": "k_face<dual5> (one launch per spatial color = 5 seed JVPs)",
+            "bound": "hbm",
+            "achieved": ach5,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ach5 / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": avg5,
+            "launches": face5_n,
+            "algorithmic_bytes_per_launch": bytes5,
+        },
+        "residual": {
+            "ms": res_ms,
+            "mcells_per_s": cells / (res_ms * 1e-3) / 1e6,
+            "achieved_gbs": ach_res,
+            "frac": ach_res / HBM_PEAK_GBS,
+            "face_kernel_avg_ms": avg64,
+        },
+        "jvp_seed_mcells_per_s": 245 * cells / (ms_step * 1e-3) / 1e6,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,91 @@
+"""ctypes binding of libbroadcast_amd.so (the C ABI of include/broadcast_amd.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object is missing (build it with
+``python -c "import __graft_entry__ as g; g.build()"``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbroadcast_amd.so")
+
+_dp = C.c_void_p
+
+
+class Phys(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("cp", "cv", "prandtl", "gam", "rgaz", "cs", "muref", "tref", "s_suth", "k2", "k4")]
+
+
+class Metrics(C.Structure):
+    _fields_ = [("nx", _dp), ("ny", _dp), ("vol", _dp), ("volf", _dp)]
+
+
+class BcOp(C.Structure):
+    _fields_ = [("kind", C.c_int), ("loc", C.c_int), ("box", C.c_int * 4), ("box2", C.c_int * 4),
+                ("tr", C.c_int * 2), ("data", _dp), ("lm", C.c_int), ("gam", C.c_double)]
+
+
+BC_INLET, BC_NOREF, BC_EXTRAP2, BC_WALL_ADIA, BC_JOIN = range(5)
+
+_SIGS = {
+    "bca_ctx_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, _dp],
+    "bca_ctx_destroy": [_dp],
+    "bca_ctx_set_stream": [_dp, _dp],
+    "bca_ctx_scratch_bytes": [_dp],
+    "bca_profile_enable": [_dp, C.c_int],
+    "bca_profile_read": [_dp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
+    "bca_geom": [_dp] * 9,
+    "bca_volf": [_dp] * 3,
+    "bca_residual_dnc5": [_dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_jvp_dnc5": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_bc_inlet": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, C.c_double],
+    "bca_bc_noref": [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_double],
+    "bca_bc_extrap2": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), C.c_int],
+    "bca_bc_wall_adia": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), C.c_double],
+    "bca_bc_join": [_dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+                    C.c_int, C.c_int],
+    "bca_norml2inf": [_dp, _dp, C.POINTER(C.c_double)],
+    "bca_norml2": [_dp, _dp, C.POINTER(C.c_double)],
+    "bca_seed": [_dp, _dp, C.c_int, C.c_int, C.c_int],
+    "bca_scatter_jv": [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_long],
+    "bca_set_bndbl": [_dp, _dp, _dp, _dp],
+    "bca_jacobian_colored": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
+                             C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
+}
+
+# Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).
+EXPORTS = sorted(list(_SIGS) + ["bca_last_error"])
+
+_lib = None
+
+
+class BcaError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: the HIP library must be built (__graft_entry__.build()); "
+                "broadcast_amd has no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, argtypes in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = C.c_long if name == "bca_ctx_scratch_bytes" else C.c_int
+        L.bca_last_error.restype = C.c_char_p
+        L.bca_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().bca_last_error().decode(errors="replace")
+        raise BcaError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,504 @@
+// C ABI of the gfx950 DNC5 path (declared in include/broadcast_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/broadcast_amd.h"
+#include "bcs.h"
+#include "misc.h"
+#include "scheme.h"
+
+using namespace bca;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define BCA_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(-2, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define BCA_LAUNCHED()                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess) return fail(-3, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Per-scalar-type scratch planes of one pipeline pass.
+template <class T>
+struct Work {
+  T* w = nullptr;
+  T* res = nullptr;
+  T* prim = nullptr;
+  T* grad = nullptr;
+  T* hn = nullptr;
+};
+
+}  // namespace
+
+// HIP-event timing of individual kernel launches on the context stream
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<std::string, size_t>> marks;  // name, index of start event
+};
+
+struct bca_ctx {
+  Mesh m;
+  hipStream_t stream = nullptr;
+  long bytes = 0;
+  KernelTimer timer;
+  Work<double> w0;
+  Work<Dual<1>> w1;
+  Work<Dual<5>> w5;
+  double* norm_part = nullptr;
+  double* norm_out = nullptr;
+};
+
+namespace {
+
+constexpr int NORM_BLOCKS = 240;
+
+template <class T> const char* tname();
+template <> const char* tname<double>() { return "f64"; }
+template <> const char* tname<Dual<1>>() { return "dual1"; }
+template <> const char* tname<Dual<5>>() { return "dual5"; }
+
+// Records a start event before and a stop event after the launch it wraps.
+struct Timed {
+  bca_ctx* c;
+  size_t idx = 0;
+  bool live = false;
+  Timed(bca_ctx* cc, const std::string& name) : c(cc) {
+    KernelTimer& t = c->timer;
+    if (!t.on) return;
+    while (t.pool.size() < t.used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      t.pool.push_back(e);
+    }
+    idx = t.used;
+    t.used += 2;
+    t.marks.emplace_back(name, idx);
+    live = hipEventRecord(t.pool[idx], c->stream) == hipSuccess;
+  }
+  ~Timed() {
+    if (live) (void)hipEventRecord(c->timer.pool[idx + 1], c->stream);
+  }
+};
+
+template <class T>
+int alloc_planes(bca_ctx* c, T** p, long count) {
+  if (*p) return 0;
+  BCA_HIP(hipMalloc((void**)p, sizeof(T) * count));
+  c->bytes += (long)sizeof(T) * count;
+  return 0;
+}
+
+template <class T>
+int ensure_work(bca_ctx* c, Work<T>& wk, bool with_state) {
+  const Mesh& m = c->m;
+  int rc = 0;
+  if ((rc = alloc_planes(c, &wk.prim, NPRIM * m.ncell))) return rc;
+  if ((rc = alloc_planes(c, &wk.grad, NGRAD * m.ncell))) return rc;
+  if ((rc = alloc_planes(c, &wk.hn, 10 * m.nnode))) return rc;
+  if (with_state) {
+    if ((rc = alloc_planes(c, &wk.w, 5 * m.ncell))) return rc;
+    if ((rc = alloc_planes(c, &wk.res, 5 * m.ncell))) return rc;
+  }
+  return 0;
+}
+
+template <class T>
+void free_work(Work<T>& wk) {
+  for (T* p : {wk.w, wk.res, wk.prim, wk.grad, wk.hn})
+    if (p) (void)hipFree(p);
+  wk = Work<T>();
+}
+
+inline dim3 grid2(int ni, int nj, dim3 b) { return dim3((ni + b.x - 1) / b.x, (nj + b.y - 1) / b.y); }
+inline unsigned grid1(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+Phys to_phys(const bca_phys* p) {
+  return Phys{p->cp, p->cv, p->prandtl, p->gam, p->rgaz, p->cs, p->muref, p->tref, p->s_suth, p->k2, p->k4};
+}
+Metric to_metric(const bca_metrics& g) { return Metric{g.nx, g.ny, g.vol, g.volf}; }
+
+int check_mesh(const Mesh& m) {
+  if (m.gh != 3) return fail(-1, "only gh = 3 (5th-order DNC) is supported");
+  if (m.im < 5 || m.jm < 5) return fail(-1, "im and jm must be >= 5 (wall stencil reaches j = 5)");
+  return 0;
+}
+
+// The residual pipeline on scalar type T; residu == nullptr stops after the faces.
+template <class T>
+int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g, const Phys& ph) {
+  const Mesh& m = c->m;
+  hipStream_t s = c->stream;
+  const std::string sfx = std::string("<") + tname<T>() + ">";
+  {
+    Timed t(c, "k_prim" + sfx);
+    k_prim<T><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, wk.prim);
+  }
+  BCA_LAUNCHED();
+  const dim3 b(64, 4);
+  {
+    Timed t(c, "k_grad" + sfx);
+    k_grad<T><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
+  }
+  BCA_LAUNCHED();
+  k_grad_ghost<T><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, wk.grad);
+  BCA_LAUNCHED();
+  const dim3 bf(64, sizeof(T) > 8 ? 1 : 4);
+  {
+    Timed t(c, "k_face" + sfx);
+    k_face<T><<<grid2(m.im + 1, m.jm + 1, bf), bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+  }
+  BCA_LAUNCHED();
+  if (residu) {
+    Timed t(c, "k_balance" + sfx);
+    k_balance<T><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, wk.hn, residu);
+  }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int box_ok(const Mesh& m, int loc, const int* bx) {
+  if (loc < 0 || loc > 3) return fail(-1, "loc must be 0..3 (Ilo, Ihi, Jlo, Jhi)");
+  const int lo = 1 - m.gh;
+  if (bx[0] < lo || bx[2] > m.im + m.gh || bx[1] < lo || bx[3] > m.jm + m.gh || bx[0] > bx[2] ||
+      bx[1] > bx[3])
+    return fail(-1, "interface box outside the block");
+  // the ghost layers written and the interior layers read must stay inside
+  const int i0 = loc == 0 ? 1 : loc == 1 ? -1 : 0, j0 = loc == 2 ? 1 : loc == 3 ? -1 : 0;
+  const int reach = m.gh;
+  for (int corner = 0; corner < 2; ++corner) {
+    const int i = corner ? bx[2] : bx[0], j = corner ? bx[3] : bx[1];
+    for (int d = -reach; d <= reach; ++d) {
+      const int ii = i + d * i0, jj = j + d * j0;
+      if (ii < lo || ii > m.im + m.gh || jj < lo || jj > m.jm + m.gh)
+        return fail(-1, "interface box too close to the block edge for gh ghost layers");
+    }
+  }
+  return 0;
+}
+
+template <class T>
+int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny) {
+  const Mesh& m = c->m;
+  hipStream_t s = c->stream;
+  const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
+  int rc = 0;
+  if (op.kind != BCA_BC_JOIN && (rc = box_ok(m, op.loc, op.box))) return rc;
+  const Side sd = decode_side(op.loc, b.imin, b.jmin, b.imax, b.jmax);
+  switch (op.kind) {
+    case BCA_BC_INLET:
+      if (!op.data || !nx || !ny) return fail(-1, "inlet needs field, nx, ny");
+      // the reference state is read at field(j, i, :) and the ghost profile at
+      // field(l, de, :): both must stay inside field(lm, gh, 5)
+      if (b.imin < 1 || b.imax > m.gh || b.jmin < 1 || b.jmax > op.lm || sd.lmax > op.lm)
+        return fail(-1, "inlet interface does not fit the field(lm, gh, 5) profile");
+      k_bc_inlet<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny, op.gam);
+      break;
+    case BCA_BC_NOREF:
+      if (!op.data || !nx || !ny) return fail(-1, "noref needs wbd, nx, ny");
+      if (op.lm < sd.lmax) return fail(-1, "wbd shorter than the interface");
+      k_bc_noref<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny, op.gam);
+      break;
+    case BCA_BC_EXTRAP2:
+      k_bc_extrap2<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, 5);
+      break;
+    case BCA_BC_WALL_ADIA:
+      k_bc_wall<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, op.gam);
+      break;
+    case BCA_BC_JOIN: {
+      const Box prr{op.box[0], op.box[1], op.box[2], op.box[3]};
+      const Box prd{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
+      const int lo = 1 - m.gh;
+      for (const Box& q : {prr, prd})
+        if (q.imin < lo || q.imax > m.im + m.gh || q.jmin < lo || q.jmax > m.jm + m.gh ||
+            q.imin > q.imax || q.jmin > q.jmax)
+          return fail(-1, "join box outside the block");
+      if (op.tr[0] != 1 || op.tr[1] != 2) return fail(-1, "batched join supports the identity transform only");
+      if (prr.imax - prr.imin != prd.imax - prd.imin || prr.jmax - prr.jmin != prd.jmax - prd.jmin)
+        return fail(-1, "join boxes differ in size");
+      const long n = (long)(prd.imax - prd.imin + 1) * (prd.jmax - prd.jmin + 1);
+      k_join<T><<<grid1(n, 256), 256, 0, s>>>(m, w, prr, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj, m.ncell);
+      break;
+    }
+    default:
+      return fail(-1, "unknown boundary-condition kind");
+  }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bca_last_error(void) { return g_err.c_str(); }
+
+int bca_ctx_create(bca_ctx** out, int im, int jm, int gh, void* stream) {
+  if (!out) return fail(-1, "null ctx pointer");
+  bca_ctx* c = new bca_ctx();
+  c->m = make_mesh(im, jm, gh);
+  c->stream = (hipStream_t)stream;
+  int rc = check_mesh(c->m);
+  if (rc) { delete c; return rc; }
+  *out = c;
+  return 0;
+}
+
+int bca_ctx_destroy(bca_ctx* c) {
+  if (!c) return 0;
+  for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+  free_work(c->w0);
+  free_work(c->w1);
+  free_work(c->w5);
+  if (c->norm_part) (void)hipFree(c->norm_part);
+  if (c->norm_out) (void)hipFree(c->norm_out);
+  delete c;
+  return 0;
+}
+
+int bca_ctx_set_stream(bca_ctx* c, void* stream) {
+  if (!c) return fail(-1, "null ctx");
+  c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+long bca_ctx_scratch_bytes(const bca_ctx* c) { return c ? c->bytes : 0; }
+
+int bca_profile_enable(bca_ctx* c, int on) {
+  if (!c) return fail(-1, "null ctx");
+  BCA_HIP(hipStreamSynchronize(c->stream));
+  c->timer.on = on != 0;
+  c->timer.used = 0;
+  c->timer.marks.clear();
+  return 0;
+}
+
+int bca_profile_read(bca_ctx* c, const char* kernel, double* total_ms, long* launches) {
+  if (!c || !kernel || !total_ms || !launches) return fail(-1, "null argument");
+  BCA_HIP(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  long n = 0;
+  for (const auto& mk : c->timer.marks) {
+    if (mk.first != kernel) continue;
+    float ms = 0.f;
+    BCA_HIP(hipEventElapsedTime(&ms, c->timer.pool[mk.second], c->timer.pool[mk.second + 1]));
+    tot += ms;
+    ++n;
+  }
+  *total_ms = tot;
+  *launches = n;
+  return 0;
+}
+
+int bca_geom(bca_ctx* c, double* x0, double* y0, double* nx, double* ny, double* xc, double* yc,
+             double* vol, double* volf) {
+  if (!c || !x0 || !y0 || !nx || !ny || !xc || !yc || !vol || !volf) return fail(-1, "null argument");
+  k_geom<<<1, 1024, 0, c->stream>>>(c->m, x0, y0, nx, ny, xc, yc, vol, volf);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_volf(bca_ctx* c, const double* vol, double* volf) {
+  if (!c || !vol || !volf) return fail(-1, "null argument");
+  const dim3 b(64, 4);
+  k_volf<<<grid2(c->m.im + 1, c->m.jm + 1, b), b, 0, c->stream>>>(c->m, vol, volf);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_residual_dnc5(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+  if (!c || !residu || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
+  int rc = ensure_work(c, c->w0, false);
+  if (rc) return rc;
+  return run_pipeline<double>(c, c->w0, w, residu, to_metric(g), to_phys(p));
+}
+
+int bca_jvp_dnc5(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
+                 bca_metrics g, const bca_phys* p) {
+  if (!c || !w || !wd || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
+  int rc = ensure_work(c, c->w1, true);
+  if (rc) return rc;
+  const Mesh& m = c->m;
+  const long n = 5 * m.ncell;
+  k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
+  BCA_LAUNCHED();
+  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, c->w1.res, to_metric(g), to_phys(p)))) return rc;
+  const dim3 b(64, 4);
+  k_unpack_interior<1><<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w1.res, residu, residud);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+static int bc_entry(bca_ctx* c, double* w, double* wd, const bca_bc_op& op, const double* nx,
+                    const double* ny) {
+  if (!c || !w) return fail(-1, "null argument");
+  if (!wd) return launch_bc<double>(c, w, op, nx, ny);
+  int rc = ensure_work(c, c->w1, true);
+  if (rc) return rc;
+  const long n = 5 * c->m.ncell;
+  k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
+  BCA_LAUNCHED();
+  if ((rc = launch_bc<Dual<1>>(c, c->w1.w, op, nx, ny))) return rc;
+  k_unpack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, c->w1.w, w, wd);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_bc_inlet(bca_ctx* c, double* w, double* wd, int loc, const int box[4], const double* field,
+                 int lm, const double* nx, const double* ny, double gam) {
+  if (!box) return fail(-1, "null box");
+  bca_bc_op op{};
+  op.kind = BCA_BC_INLET; op.loc = loc; std::memcpy(op.box, box, sizeof(op.box));
+  op.data = field; op.lm = lm; op.gam = gam;
+  if (c && lm < (loc <= 1 ? box[3] : box[2])) return fail(-1, "field shorter than the interface");
+  return bc_entry(c, w, wd, op, nx, ny);
+}
+
+int bca_bc_noref(bca_ctx* c, double* w, double* wd, const double* wbd, int lm, int loc, const int box[4],
+                 const double* nx, const double* ny, double gam) {
+  if (!box) return fail(-1, "null box");
+  bca_bc_op op{};
+  op.kind = BCA_BC_NOREF; op.loc = loc; std::memcpy(op.box, box, sizeof(op.box));
+  op.data = wbd; op.lm = lm; op.gam = gam;
+  return bc_entry(c, w, wd, op, nx, ny);
+}
+
+int bca_bc_extrap2(bca_ctx* c, double* w, double* wd, int loc, const int box[4], int em) {
+  if (!box) return fail(-1, "null box");
+  if (em != 5) return fail(-1, "only em = 5 is supported");
+  bca_bc_op op{};
+  op.kind = BCA_BC_EXTRAP2; op.loc = loc; std::memcpy(op.box, box, sizeof(op.box));
+  return bc_entry(c, w, wd, op, nullptr, nullptr);
+}
+
+int bca_bc_wall_adia(bca_ctx* c, double* w, double* wd, int loc, const int box[4], double gam) {
+  if (!box) return fail(-1, "null box");
+  bca_bc_op op{};
+  op.kind = BCA_BC_WALL_ADIA; op.loc = loc; std::memcpy(op.box, box, sizeof(op.box)); op.gam = gam;
+  return bc_entry(c, w, wd, op, nullptr, nullptr);
+}
+
+int bca_bc_join(bca_ctx* c, double* a, const int prr[4], const int prd[4], const int tr[2], int em,
+                int ni_arr, int nj_arr) {
+  if (!c || !a || !prr || !prd || !tr) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int lo = 1 - m.gh, hi_i = ni_arr - m.gh, hi_j = nj_arr - m.gh;
+  for (const int* bx : {prr, prd})
+    if (bx[0] < lo || bx[2] > hi_i || bx[1] < lo || bx[3] > hi_j || bx[0] > bx[2] || bx[1] > bx[3])
+      return fail(-1, "join box outside the array");
+  const int idir = tr[0] >= 0 ? tr[0] : -tr[0], jdir = tr[1] >= 0 ? tr[1] : -tr[1];
+  if (idir < 1 || idir > 2 || jdir < 1 || jdir > 2 || idir == jdir) return fail(-1, "bad transform");
+  const int pd[2][2] = {{prd[0], prd[2]}, {prd[1], prd[3]}};
+  const int cnt_i = pd[idir - 1][1] - pd[idir - 1][0] + 1, cnt_j = pd[jdir - 1][1] - pd[jdir - 1][0] + 1;
+  if (prr[0] + cnt_i - 1 > hi_i || prr[1] + cnt_j - 1 > hi_j) return fail(-1, "join receiver too small");
+  const Box rb{prr[0], prr[1], prr[2], prr[3]}, db{prd[0], prd[1], prd[2], prd[3]};
+  const long plane = (long)ni_arr * nj_arr;
+  k_join<double><<<grid1((long)cnt_i * cnt_j, 256), 256, 0, c->stream>>>(m, a, rb, db, tr[0], tr[1], em,
+                                                                          ni_arr, nj_arr, plane);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+static int norms(bca_ctx* c, const double* rhs, double* out10, bool inf) {
+  if (!c || !rhs || !out10) return fail(-1, "null argument");
+  if (!c->norm_part) {
+    BCA_HIP(hipMalloc((void**)&c->norm_part, sizeof(double) * 10 * NORM_BLOCKS));
+    BCA_HIP(hipMalloc((void**)&c->norm_out, sizeof(double) * 10));
+    c->bytes += sizeof(double) * (10 * NORM_BLOCKS + 10);
+  }
+  k_norm_partial<<<NORM_BLOCKS, NORM_THREADS, 0, c->stream>>>(c->m, rhs, c->norm_part);
+  BCA_LAUNCHED();
+  k_norm_final<<<1, 64, 0, c->stream>>>(c->norm_part, NORM_BLOCKS, c->norm_out);
+  BCA_LAUNCHED();
+  double h[10];
+  BCA_HIP(hipMemcpyAsync(h, c->norm_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  BCA_HIP(hipStreamSynchronize(c->stream));
+  for (int e = 0; e < 5; ++e) {
+    out10[e] = h[e];
+    out10[5 + e] = inf ? h[5 + e] : h[e] / ((double)c->m.im * c->m.jm);
+  }
+  return 0;
+}
+
+int bca_norml2inf(bca_ctx* c, const double* rhs, double* out10) { return norms(c, rhs, out10, true); }
+int bca_norml2(bca_ctx* c, const double* rhs, double* out10) { return norms(c, rhs, out10, false); }
+
+int bca_seed(bca_ctx* c, double* wd, int mv, int l, int k) {
+  if (!c || !wd) return fail(-1, "null argument");
+  const int step = 2 * c->m.gh + 1;
+  if (mv < 0 || mv > 4 || l < 0 || l >= step || k < 0 || k >= step) return fail(-1, "seed index out of range");
+  k_testvector<<<grid1(5 * c->m.ncell, 256), 256, 0, c->stream>>>(c->m, wd, mv, l, k);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_scatter_jv(bca_ctx* c, double* jac, int* ia, int* ja, const double* resd, int mv, int l, int k,
+                   const double* coefdiag, int withjn, long nbentry) {
+  if (!c || !jac || !ia || !ja || !resd) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (mv < 0 || mv > 4 || l < 0 || l >= step || k < 0 || k >= step) return fail(-1, "seed index out of range");
+  if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
+  k_scatter_jv<<<grid1(5L * m.im * m.jm, 256), 256, 0, c->stream>>>(m, jac, ia, ja, resd, mv, l, k,
+                                                                     coefdiag, withjn);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_set_bndbl(bca_ctx* c, const double* w, double* field, double* wbd) {
+  if (!c || !w || !field || !wbd) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  k_set_bndbl<<<grid1((long)m.jm * m.gh + m.im + m.gh, 256), 256, 0, c->stream>>>(m, w, field, wbd);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
+                         const bca_bc_op* ops, int nops, const double* coefdiag, int withjn,
+                         double* jac, int* ia, int* ja, long nbentry, int write_index) {
+  if (!c || !w || !p || !jac || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+  if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
+  int rc = ensure_work(c, c->w5, true);
+  if (rc) return rc;
+  const Metric gm = to_metric(g);
+  const Phys ph = to_phys(p);
+  const dim3 b(64, 4);
+  for (int l = 0; l < step; ++l)
+    for (int k = 0; k < step; ++k) {
+      k_seed_color<<<grid1(5 * m.ncell, 256), 256, 0, c->stream>>>(m, w, c->w5.w, l, k);
+      BCA_LAUNCHED();
+      for (int q = 0; q < nops; ++q)
+        if ((rc = launch_bc<Dual<5>>(c, c->w5.w, ops[q], g.nx, g.ny))) return rc;
+      if ((rc = run_pipeline<Dual<5>>(c, c->w5, c->w5.w, nullptr, gm, ph))) return rc;
+      {
+        Timed t(c, "k_balance_scatter<dual5>");
+        k_balance_scatter<<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w5.hn, jac, ia, ja, l, k,
+                                                                     coefdiag, withjn, write_index);
+      }
+      BCA_LAUNCHED();
+    }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+"""Device-resident driver of the FP64 residual / JVP / Jacobian path.
+
+``Flow`` keeps a case's geometry, boundary data and boundary-condition
+sequence in HBM and runs the reference driver's per-iteration work on the
+MI355X without host round trips: the BC sequence and residual of
+BROADCAST_npz.py:1018-1031 (cylinder.py:853-868 for the O-mesh), the tangent
+BCs + JVP of :1079-1085, and the whole 245-seed colored Jacobian assembly of
+:1068-1127 as one call into the C ABI.
+
+All device arrays are flat float64 tensors holding the reference's Fortran
+layout (ghost frame included); ``to_device``/``to_host`` convert from/to numpy
+F-order arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import device as dev
+from .cases import Case, join_boxes
+
+
+def to_device(a):
+    t = dev.require_gpu()
+    a = np.asfortranarray(a)
+    return t.from_numpy(a.reshape(-1, order="F").copy()).to("cuda")
+
+
+def to_host(t, shape):
+    return np.asarray(t.detach().cpu().numpy()).reshape(shape, order="F")
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Flow:
+    def __init__(self, case: Case):
+        self.case = case
+        self.t = dev.require_gpu()
+        im, jm, gh = case.im, case.jm, case.gh
+        self.im, self.jm, self.gh = im, jm, gh
+        self.ni, self.nj = case.ni, case.nj
+        self.shape_w = (self.ni, self.nj, 5)
+        self.ctx = dev.context(im, jm, gh)
+        self.L = _lib.lib()
+        args = case.phys.scheme_args()
+        self.phys = dev.phys(*args)
+        self.gam = float(args[3])
+        self._geometry()
+        self._plan()
+
+    # ------------------------------------------------------------ geometry
+    def _geometry(self):
+        c, t = self.case, self.t
+        ni, nj = self.ni, self.nj
+        z = lambda n: t.zeros(n, dtype=t.float64, device="cuda")
+        self.x0 = to_device(c.x0)
+        self.y0 = to_device(c.y0)
+        self.nx = z((ni + 1) * (nj + 1) * 2)
+        self.ny = z((ni + 1) * (nj + 1) * 2)
+        self.xc, self.yc, self.vol = z(ni * nj), z(ni * nj), z(ni * nj)
+        self.volf = z(ni * nj * 2)
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_geom(h, *map(_ptr, (self.x0, self.y0, self.nx, self.ny, self.xc, self.yc,
+                                                  self.vol, self.volf))), "bca_geom")
+        if c.kind == "cyl":
+            # periodic ghost metrics (cylinder.py:496-527)
+            im, jm, gh = self.im, self.jm, self.gh
+            (prr1, prd2, tr), (prr2, prd1, _) = join_boxes(im, jm, gh)
+            trc = (C.c_int * 2)(1, 2)
+            for arr in (self.xc, self.yc, self.vol):
+                for a, b in ((prr1, prd2), (prr2, prd1)):
+                    _lib.check(self.L.bca_bc_join(h, _ptr(arr), dev.int4(a), dev.int4(b), trc, 1, ni, nj),
+                               "bca_bc_join")
+            jmin, jmax = 1 - gh, jm + gh
+            box = lambda a, b, cc, d: np.array([[a, b], [cc, d]])
+            prr1g, prd1g = box(im + 2, jmin, im + 1 + gh, jmax + 1), box(im + 1 - gh, jmin, im, jmax + 1)
+            prr2g, prd2g = box(1 - gh, jmin, 0, jmax + 1), box(2, jmin, 1 + gh, jmax + 1)
+            for arr in (self.nx, self.ny):
+                for a, b in ((prr1g, prd2g), (prr2g, prd1g)):
+                    _lib.check(self.L.bca_bc_join(h, _ptr(arr), dev.int4(a), dev.int4(b), trc, 2, ni + 1,
+                                                  nj + 1), "bca_bc_join")
+            _lib.check(self.L.bca_volf(h, _ptr(self.vol), _ptr(self.volf)), "bca_volf")
+        self.metrics = _lib.Metrics(_ptr(self.nx), _ptr(self.ny), _ptr(self.vol), _ptr(self.volf))
+
+    # ------------------------------------------------------------ BC sequence
+    def _op(self, kind, loc=0, box=None, box2=None, data=None, lm=0):
+        op = _lib.BcOp()
+        op.kind, op.loc, op.gam = kind, loc, self.gam
+        if box is not None:
+            op.box[:] = list(dev.int4(box))
+        if box2 is not None:
+            op.box2[:] = list(dev.int4(box2))
+        op.tr[:] = [1, 2]
+        op.data = None if data is None else data.data_ptr()
+        op.lm = lm
+        return op
+
+    def _plan(self):
+        c = self.case
+        it = c.interf
+        ops = []
+        if c.field is not None:
+            self.field = to_device(c.field)
+        self.wbd = to_device(c.wbd)
+        if c.kind == "bl":
+            # BROADCAST_npz.py:1018-1021: inflow, far field, outflow, wall
+            ops.append(self._op(_lib.BC_INLET, 0, it["Ilo"], data=self.field, lm=c.field.shape[0]))
+            ops.append(self._op(_lib.BC_NOREF, 3, it["Jhi"], data=self.wbd, lm=c.wbd.shape[0]))
+            ops.append(self._op(_lib.BC_EXTRAP2, 1, it["Ihi"]))
+            ops.append(self._op(_lib.BC_WALL_ADIA, 2, it["Jlo"]))
+        else:
+            # handleBC.sortBC order for the O-mesh: far field, wall, then the
+            # periodic join (both directions, applied by each join interface)
+            ops.append(self._op(_lib.BC_NOREF, 3, it["Jhi"], data=self.wbd, lm=c.wbd.shape[0]))
+            ops.append(self._op(_lib.BC_WALL_ADIA, 2, it["Jlo"]))
+            (prr1, prd2, _), (prr2, prd1, _) = join_boxes(self.im, self.jm, self.gh)
+            for _ in range(2):
+                ops.append(self._op(_lib.BC_JOIN, 0, prr1, prd2))
+                ops.append(self._op(_lib.BC_JOIN, 0, prr2, prd1))
+        self.ops = (_lib.BcOp * len(ops))(*ops)
+        self.nops = len(ops)
+
+    def apply_bcs(self, w, wd=None):
+        """Ghost filling in place (w, and wd with the linearised BCs)."""
+        h = self.ctx.sync_stream()
+        L = self.L
+        for op in self.ops:
+            wp, wdp = _ptr(w), _ptr(wd)
+            box = (C.c_int * 4)(*op.box)
+            if op.kind == _lib.BC_INLET:
+                rc = L.bca_bc_inlet(h, wp, wdp, op.loc, box, op.data, op.lm, _ptr(self.nx), _ptr(self.ny), op.gam)
+            elif op.kind == _lib.BC_NOREF:
+                rc = L.bca_bc_noref(h, wp, wdp, op.data, op.lm, op.loc, box, _ptr(self.nx), _ptr(self.ny), op.gam)
+            elif op.kind == _lib.BC_EXTRAP2:
+                rc = L.bca_bc_extrap2(h, wp, wdp, op.loc, box, 5)
+            elif op.kind == _lib.BC_WALL_ADIA:
+                rc = L.bca_bc_wall_adia(h, wp, wdp, op.loc, box, op.gam)
+            else:
+                box2 = (C.c_int * 4)(*op.box2)
+                trc = (C.c_int * 2)(*op.tr)
+                rc = 0
+                for arr in ([wd] if wd is not None else []) + [w]:
+                    rc = rc or L.bca_bc_join(h, _ptr(arr), box, box2, trc, 5, self.ni, self.nj)
+            _lib.check(rc, "boundary condition")
+
+    # ------------------------------------------------------------ operators
+    def new_state(self):
+        return self.t.zeros(self.ni * self.nj * 5, dtype=self.t.float64, device="cuda")
+
+    def residual(self, w, res=None):
+        res = self.new_state() if res is None else res
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_residual_dnc5(h, _ptr(res), _ptr(w), self.metrics, C.byref(self.phys)),
+                   "bca_residual_dnc5")
+        return res
+
+    def jvp(self, w, wd, res=None, resd=None):
+        """Tangent BCs on (w, wd) then res, resd = R(w), dR/dw . wd."""
+        res = self.new_state() if res is None else res
+        resd = self.new_state() if resd is None else resd
+        self.apply_bcs(w, wd)
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_jvp_dnc5(h, _ptr(res), _ptr(resd), _ptr(w), _ptr(wd), self.metrics,
+                                       C.byref(self.phys)), "bca_jvp_dnc5")
+        return res, resd
+
+    def norms(self, res):
+        out = (C.c_double * 10)()
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_norml2inf(h, _ptr(res), out), "bca_norml2inf")
+        o = np.array(out[:])
+        return o[:5], o[5:]
+
+    @property
+    def nbentry(self):
+        s = 2 * self.gh + 1
+        return self.im * self.jm * s * s * 25
+
+    def jacobian_buffers(self):
+        t = self.t
+        n = self.nbentry
+        return (t.zeros(n, dtype=t.float64, device="cuda"), t.zeros(n, dtype=t.int32, device="cuda"),
+                t.zeros(n, dtype=t.int32, device="cuda"))
+
+    def jacobian_colored(self, w, coefdiag=None, bufs=None, write_index=True):
+        """The reference's 245-seed assembly into its COO layout (jac, ia, ja)
+        on the device: -dR/dw (+ coefdiag on the diagonal); w must carry its
+        ghost frame (BCs applied)."""
+        jac, ia, ja = self.jacobian_buffers() if bufs is None else bufs
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_jacobian_colored(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops,
+                                               self.nops, _ptr(coefdiag), int(self.case.kind == "cyl"),
+                                               _ptr(jac), _ptr(ia), _ptr(ja), self.nbentry,
+                                               int(write_index)), "bca_jacobian_colored")
+        return jac, ia, ja

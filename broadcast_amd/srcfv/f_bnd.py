@@ -1,0 +1,37 @@
+"""f_bnd: ghost-frame boundary conditions (srcfv/borders, srcfv/compile_borders.py:35).
+
+Call forms of BROADCAST_npz.py:1018-1021 and handleBC.py:184-240; arrays in the
+reference's layout (numpy F-order, staged, or HIP torch tensors, zero copy)."""
+from . import _bc
+
+
+def bc_supandsubinlet_2d(w, loc, interf, field, nx, ny, gam, im, jm, lm=None, gh=None):
+    """srcfv/borders/bc_supandsubinlet.F90:2-171"""
+    _bc.inlet(w, None, loc, interf, field, nx, ny, gam, im, jm, lm, gh)
+
+
+def bc_no_reflexion_2d(w, wbd, loc, interf, nx, ny, gam, gh, im, jm, lm=None):
+    """srcfv/borders/bc_no_reflexion.F90:8-152"""
+    _bc.noref(w, None, wbd, loc, interf, nx, ny, gam, gh, im, jm, lm)
+
+
+def bc_extrapolate_o2_2d(w, loc, interf, im, jm, gh, em=None):
+    """srcfv/borders/bc_extrapolate.F90:39-71"""
+    _bc.extrap2(w, None, loc, interf, im, jm, gh, em)
+
+
+def bc_wall_viscous_adia_2d(w, loc, gam, interf, gh, im, jm):
+    """srcfv/borders/bc_wall_viscous.F90:2-99"""
+    _bc.wall(w, None, loc, gam, interf, gh, im, jm)
+
+
+def jn_match_2d(wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d, gh3d, gh4d, imd, jmd,
+                tr, em=None):
+    """srcfv/borders/jn_match.F90:3-66 (single-block periodic join)"""
+    _bc.join(wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d, gh3d, gh4d, imd, jmd, tr, em)
+
+
+def jn_match_geom_2d(wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d, gh3d, gh4d, imd,
+                     jmd, tr):
+    """srcfv/prepro/jn_match_geom.f90: the same copy on a 2-D (single plane) array"""
+    _bc.join(wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d, gh3d, gh4d, imd, jmd, tr, 1)

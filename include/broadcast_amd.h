@@ -1,0 +1,158 @@
+/*
+ * broadcast_amd -- C ABI of the MI355X (gfx950) FP64 residual / JVP path of
+ * BROADCAST's DNC5 finite-volume scheme.
+ *
+ * Every array argument is a DEVICE pointer (HIP global memory) laid out
+ * exactly as the reference's Fortran dummies: column-major, lower bound 1-gh
+ * on both axes, components as trailing planes.  With ni = im+2gh,
+ * nj = jm+2gh:
+ *   cell arrays  w, residu, wd, ...   ni x nj x 5      (float64)
+ *   node arrays  x0, y0               (ni+1) x (nj+1)
+ *   face arrays  nx, ny               (ni+1) x (nj+1) x 2
+ *   vol, xc, yc                       ni x nj
+ *   volf                              ni x nj x 2
+ * Work is enqueued on the context's HIP stream; calls return 0 on success and
+ * a negative code on error (bca_last_error() gives the message).  The caller
+ * owns every buffer; the context owns its scratch.  Calls on one context are
+ * not thread-safe; separate contexts are independent.
+ *
+ * Side codes (loc): 0 = 'Ilo', 1 = 'Ihi', 2 = 'Jlo', 3 = 'Jhi'.
+ * Boxes: {imin, jmin, imax, jmax} as the reference's interf(2,2)
+ * (BROADCAST_npz.py:706-731).
+ */
+#ifndef BROADCAST_AMD_H
+#define BROADCAST_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bca_ctx bca_ctx;
+
+/* Thermodynamic and scheme constants in the positional order of
+ * flux_num_dnc5_2d (srcfv/rhs/flux_num_dnc5.F90:7-8). */
+typedef struct {
+  double cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4;
+} bca_phys;
+
+/* Metrics produced by bca_geom (nx, ny, vol, volf; device pointers). */
+typedef struct {
+  const double *nx, *ny, *vol, *volf;
+} bca_metrics;
+
+/* One ghost-filling operation of a boundary-condition sequence. */
+enum { BCA_BC_INLET = 0, BCA_BC_NOREF = 1, BCA_BC_EXTRAP2 = 2, BCA_BC_WALL_ADIA = 3, BCA_BC_JOIN = 4 };
+typedef struct {
+  int kind;            /* BCA_BC_* */
+  int loc;             /* side code */
+  int box[4];          /* interface box; for BCA_BC_JOIN the receiving box prr */
+  int box2[4];         /* BCA_BC_JOIN: donor box prd */
+  int tr[2];           /* BCA_BC_JOIN: index transform */
+  const double *data;  /* INLET: field (lm x gh x 5); NOREF: wbd (lm x 5); device */
+  int lm;              /* leading dimension of data */
+  double gam;
+} bca_bc_op;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: holds the mesh dimensions, the stream and the
+ * scratch planes that the reference keeps as automatic arrays
+ * (srcfv/rhs/flux_num_dnc5.F90:29-37). */
+int bca_ctx_create(bca_ctx **ctx, int im, int jm, int gh, void *hip_stream);
+int bca_ctx_destroy(bca_ctx *ctx);
+int bca_ctx_set_stream(bca_ctx *ctx, void *hip_stream);
+const char *bca_last_error(void);
+/* Number of bytes of device scratch the context currently holds. */
+long bca_ctx_scratch_bytes(const bca_ctx *ctx);
+/* Kernel timing with HIP events recorded on the context stream around each
+ * pipeline launch (no reference counterpart; the reference times with wall
+ * clocks, BROADCAST_npz.py:1003-1007).  enable resets the record; read sums
+ * the launches of one kernel, named "k_face<f64>", "k_face<dual1>",
+ * "k_face<dual5>", "k_prim<..>", "k_grad<..>", "k_balance<..>",
+ * "k_balance_scatter<dual5>". */
+int bca_profile_enable(bca_ctx *ctx, int on);
+int bca_profile_read(bca_ctx *ctx, const char *kernel, double *total_ms, long *launches);
+
+/* ---- f_geom -------------------------------------------------------------- */
+/* computegeom_2d(x0,y0,nx,ny,xc,yc,vol,volf,im,jm,gh)
+ * replaces srcfv/geom/computegeom.F90:3-104 (x0, y0 ghost nodes are
+ * overwritten by extrapolation, as in the reference). */
+int bca_geom(bca_ctx *ctx, double *x0, double *y0, double *nx, double *ny, double *xc,
+             double *yc, double *vol, double *volf);
+/* volf = 2/(vol_L + vol_R) on faces 1..im+1 x 1..jm+1 from the current vol
+ * (the O-mesh recomputation of cylinder.py:524-527). */
+int bca_volf(bca_ctx *ctx, const double *vol, double *volf);
+
+/* ---- f_sch --------------------------------------------------------------- */
+/* flux_num_dnc5_2d(residu,w,x0,y0,nx,ny,xc,yc,vol,volf,gh,cp,...,k4,im,jm)
+ * replaces srcfv/rhs/flux_num_dnc5.F90:7-226.  Writes residu on the interior
+ * only (ghosts untouched); x0, y0, xc, yc are not needed (never referenced by
+ * the reference routine). */
+int bca_residual_dnc5(bca_ctx *ctx, double *residu, const double *w, bca_metrics g,
+                      const bca_phys *phys);
+
+/* ---- f_lin --------------------------------------------------------------- */
+/* flux_num_dnc5_2d_d(residu,residud,w,wd,...)
+ * replaces srcfv/tangent/flux_num_dnc5_d.f90:15-3870: residu and
+ * residud = dR/dw . wd on the interior (wd's ghosts as given). */
+int bca_jvp_dnc5(bca_ctx *ctx, double *residu, double *residud, const double *w,
+                 const double *wd, bca_metrics g, const bca_phys *phys);
+
+/* ---- f_bnd (wd == NULL) and f_lin (wd != NULL: linearised BC, w and wd
+ * both updated, as the TAPENADE _d routines) -------------------------------- */
+/* bc_supandsubinlet_2d[_d]  srcfv/borders/bc_supandsubinlet.F90:2-171,
+ *                           srcfv/tangent/bc_supandsubinlet_d.f90:11 */
+int bca_bc_inlet(bca_ctx *ctx, double *w, double *wd, int loc, const int box[4],
+                 const double *field, int lm, const double *nx, const double *ny, double gam);
+/* bc_no_reflexion_2d[_d]    srcfv/borders/bc_no_reflexion.F90:8-152,
+ *                           srcfv/tangent/bc_no_reflexion_d.f90:16 */
+int bca_bc_noref(bca_ctx *ctx, double *w, double *wd, const double *wbd, int lm, int loc,
+                 const int box[4], const double *nx, const double *ny, double gam);
+/* bc_extrapolate_o2_2d[_d]  srcfv/borders/bc_extrapolate.F90:39-71,
+ *                           srcfv/tangent/bc_extrapolateo2_d.f90:18 */
+int bca_bc_extrap2(bca_ctx *ctx, double *w, double *wd, int loc, const int box[4], int em);
+/* bc_wall_viscous_adia_2d[_d] srcfv/borders/bc_wall_viscous.F90:2-99,
+ *                           srcfv/tangent/bc_wall_viscous_d.f90:11 */
+int bca_bc_wall_adia(bca_ctx *ctx, double *w, double *wd, int loc, const int box[4], double gam);
+/* jn_match_2d               srcfv/borders/jn_match.F90:3-66 (same array as
+ * receiver and donor, as the O-mesh periodic join uses it); the array has
+ * leading dimensions ni_arr x nj_arr (cells: ni x nj, faces: ni+1 x nj+1) and
+ * em planes. */
+int bca_bc_join(bca_ctx *ctx, double *a, const int prr[4], const int prd[4], const int tr[2],
+                int em, int ni_arr, int nj_arr);
+
+/* ---- f_norm -------------------------------------------------------------- */
+/* compute_norml2inf(norm,ninf,rhs,im,jm,gh)  srcfv/norm.F90:34-77
+ * compute_norml2(norm,nmoy,rhs,im,jm,gh)     srcfv/norm.F90:2-32
+ * out[0..4] = norm, out[5..9] = ninf (or nmoy); out is HOST memory and the
+ * call synchronises the context stream. */
+int bca_norml2inf(bca_ctx *ctx, const double *rhs, double *out10);
+int bca_norml2(bca_ctx *ctx, const double *rhs, double *out10);
+
+/* ---- f_misc -------------------------------------------------------------- */
+/* testvector(wd,m,l,k,gh,im,jm)  misc/ComputeJacobian.f90:357-374 */
+int bca_seed(bca_ctx *ctx, double *wd, int m, int l, int k);
+/* computejacobianfromjv[_relaxed][_withjn](jac,ia,ja,resd,m,l,k,gh,im,jm,
+ * nbentry[,coefdiag])  misc/ComputeJacobian.f90:292-355, 503-570, 847-926:
+ * coefdiag == NULL selects the plain variant, withjn the join-aware one.
+ * jac/ia/ja have nbentry = im*jm*(2gh+1)^2*25 entries; coefdiag is im x jm. */
+int bca_scatter_jv(bca_ctx *ctx, double *jac, int *ia, int *ja, const double *resd, int m, int l,
+                   int k, const double *coefdiag, int withjn, long nbentry);
+/* set_bndbl_2d(w,field,wbd,im,jm,gh)  set_bnd.f90:2-24 */
+int bca_set_bndbl(bca_ctx *ctx, const double *w, double *field, double *wbd);
+
+/* ---- batched colored Jacobian --------------------------------------------- */
+/* The whole seed loop of BROADCAST_npz.py:1068-1127 (cylinder.py:1107-1180
+ * for the O-mesh): for every seed (m, l, k) of testvector, the linearised BC
+ * sequence `ops` (applied in order) + the tangent residual + the scatter of
+ * computejacobianfromjv[_relaxed][_withjn], into the same COO arrays the
+ * reference loop fills.  The 5 variables m of one spatial color are carried
+ * as 5 tangent directions of one pass.  write_index = 0 skips ia/ja (they
+ * depend on the mesh only). */
+int bca_jacobian_colored(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
+                         const bca_bc_op *ops, int nops, const double *coefdiag, int withjn,
+                         double *jac, int *ia, int *ja, long nbentry, int write_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BROADCAST_AMD_H */
