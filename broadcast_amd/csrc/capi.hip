@@ -9,6 +9,7 @@
 
 #include "../../include/broadcast_amd.h"
 #include "bcs.h"
+#include "faces.h"
 #include "misc.h"
 #include "scheme.h"
 
