@@ -20,17 +20,40 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    src = os.path.join(HERE, "csrc", "capi.hip")
-    out = os.path.join(HERE, "libbroadcast_amd.so")
-    tmp = out + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, src]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def build(verbose=False, jobs=None):
+    """Compile the C-ABI object and the five face-block variants in parallel,
+    then link libbroadcast_amd.so."""
+    csrc = os.path.join(HERE, "csrc")
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    units = [("capi", os.path.join(csrc, "capi.hip"), [])]
+    units += [(f"blocks{v}", os.path.join(csrc, "blocks.hip"), [f"-DBCA_VARIANT={v}"]) for v in range(5)]
+    procs = []
+    for name, src, defs in units:
+        obj = os.path.join(objdir, name + ".o")
+        cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], *defs, "-c", "-o", obj, src]
+        procs.append((name, obj, _run(cmd, verbose)))
+    failed = []
+    for name, obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append(name)
+            sys.stderr.write(out)
+    if failed:
+        raise RuntimeError(f"hipcc failed for {failed}")
+    out = os.path.join(HERE, "libbroadcast_amd.so")
+    tmp = out + ".tmp"
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[o for _, o, _ in procs]]
+    r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for libbroadcast_amd.so")
+        raise RuntimeError("link failed for libbroadcast_amd.so")
     os.replace(tmp, out)
     return out
 

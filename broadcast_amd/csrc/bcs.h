@@ -46,9 +46,10 @@ struct StateView {
 // the Roe-state pressure lacks the 1/2 (:49), velo2 uses ww unsquared (:69),
 // the ghost spanwise momentum is field(j, de, 4) (:160).
 template <class T>
-__global__ void k_bc_inlet(Mesh m, T* __restrict__ wp, int loc, Box b, const double* __restrict__ field,
+__global__ void k_bc_inlet(Mesh m, T* __restrict__ wbase, int loc, Box b, const double* __restrict__ field,
                            int lm, const double* __restrict__ nxp, const double* __restrict__ nyp,
-                           double gam) {
+                           double gam, long cstride) {
+  T* wp = wbase + (long)blockIdx.y * cstride;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > s.lmax) return;
@@ -140,9 +141,10 @@ __global__ void k_bc_inlet(Mesh m, T* __restrict__ wp, int loc, Box b, const dou
 
 // Non-reflecting far field against the reference state wbd(l, :).
 template <class T>
-__global__ void k_bc_noref(Mesh m, T* __restrict__ wp, const double* __restrict__ wbd, int lm, int loc,
+__global__ void k_bc_noref(Mesh m, T* __restrict__ wbase, const double* __restrict__ wbd, int lm, int loc,
                            Box b, const double* __restrict__ nxp, const double* __restrict__ nyp,
-                           double gam) {
+                           double gam, long cstride) {
+  T* wp = wbase + (long)blockIdx.y * cstride;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > s.lmax) return;
@@ -204,7 +206,8 @@ __global__ void k_bc_noref(Mesh m, T* __restrict__ wp, const double* __restrict_
 }
 
 template <class T>
-__global__ void k_bc_extrap2(Mesh m, T* __restrict__ wp, int loc, Box b, int em) {
+__global__ void k_bc_extrap2(Mesh m, T* __restrict__ wbase, int loc, Box b, int em, long cstride) {
+  T* wp = wbase + (long)blockIdx.y * cstride;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > s.lmax) return;
@@ -221,7 +224,8 @@ __global__ void k_bc_extrap2(Mesh m, T* __restrict__ wp, int loc, Box b, int em)
 // pressure and the isentropic density, the deeper ones mirror interior cells
 // with reversed velocity.
 template <class T>
-__global__ void k_bc_wall(Mesh m, T* __restrict__ wp, int loc, Box b, double gam) {
+__global__ void k_bc_wall(Mesh m, T* __restrict__ wbase, int loc, Box b, double gam, long cstride) {
+  T* wp = wbase + (long)blockIdx.y * cstride;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l > s.lmax) return;
@@ -263,8 +267,9 @@ __global__ void k_bc_wall(Mesh m, T* __restrict__ wp, int loc, Box b, double gam
 // prr/prd: {imin, jmin, imax, jmax}; tr: the index transform (tr[0] axis and
 // sign of the receiving i, tr[1] of j).  em components.
 template <class T>
-__global__ void k_join(Mesh m, T* __restrict__ wp, Box prr, Box prd, int tr0, int tr1, int em,
-                       int ni_arr, int nj_arr, long plane) {
+__global__ void k_join(Mesh m, T* __restrict__ wbase, Box prr, Box prd, int tr0, int tr1, int em,
+                       int ni_arr, int nj_arr, long plane, long cstride) {
+  T* wp = wbase + (long)blockIdx.y * cstride;
   const int istep = tr0 >= 0 ? 1 : -1, jstep = tr1 >= 0 ? 1 : -1;
   const int idir = tr0 >= 0 ? tr0 : -tr0, jdir = tr1 >= 0 ? tr1 : -tr1;
   const int plo[2] = {prd.imin, prd.jmin}, phi[2] = {prd.imax, prd.jmax};

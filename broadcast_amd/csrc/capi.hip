@@ -10,6 +10,8 @@
 #include "../../include/broadcast_amd.h"
 #include "bcs.h"
 #include "faces.h"
+#include "blocks.h"
+#include "jacobian.h"
 #include "misc.h"
 #include "scheme.h"
 
@@ -63,9 +65,12 @@ struct bca_ctx {
   KernelTimer timer;
   Work<double> w0;
   Work<Dual<1>> w1;
-  Work<Dual<5>> w5;
   double* norm_part = nullptr;
   double* norm_out = nullptr;
+  double* blki = nullptr;   // face-local Jacobian blocks, i-faces, one plane per offset
+  double* blkj = nullptr;   // j-faces
+  double* ghostT = nullptr; // ghost tangents per color
+  Dual<5>* wcol = nullptr;  // seeded states of the 49 colors (frames only)
 };
 
 namespace {
@@ -197,7 +202,8 @@ int box_ok(const Mesh& m, int loc, const int* bx) {
 }
 
 template <class T>
-int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny) {
+int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny, int ncolor = 1,
+              long cstride = 0) {
   const Mesh& m = c->m;
   hipStream_t s = c->stream;
   const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
@@ -211,18 +217,20 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
       // field(l, de, :): both must stay inside field(lm, gh, 5)
       if (b.imin < 1 || b.imax > m.gh || b.jmin < 1 || b.jmax > op.lm || sd.lmax > op.lm)
         return fail(-1, "inlet interface does not fit the field(lm, gh, 5) profile");
-      k_bc_inlet<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny, op.gam);
+      k_bc_inlet<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny,
+                                                                   op.gam, cstride);
       break;
     case BCA_BC_NOREF:
       if (!op.data || !nx || !ny) return fail(-1, "noref needs wbd, nx, ny");
       if (op.lm < sd.lmax) return fail(-1, "wbd shorter than the interface");
-      k_bc_noref<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny, op.gam);
+      k_bc_noref<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny,
+                                                                   op.gam, cstride);
       break;
     case BCA_BC_EXTRAP2:
-      k_bc_extrap2<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, 5);
+      k_bc_extrap2<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, 5, cstride);
       break;
     case BCA_BC_WALL_ADIA:
-      k_bc_wall<T><<<grid1(sd.lmax, 64), 64, 0, s>>>(m, w, op.loc, b, op.gam);
+      k_bc_wall<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, op.gam, cstride);
       break;
     case BCA_BC_JOIN: {
       const Box prr{op.box[0], op.box[1], op.box[2], op.box[3]};
@@ -236,7 +244,8 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
       if (prr.imax - prr.imin != prd.imax - prd.imin || prr.jmax - prr.jmin != prd.jmax - prd.jmin)
         return fail(-1, "join boxes differ in size");
       const long n = (long)(prd.imax - prd.imin + 1) * (prd.jmax - prd.jmin + 1);
-      k_join<T><<<grid1(n, 256), 256, 0, s>>>(m, w, prr, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj, m.ncell);
+      k_join<T><<<dim3(grid1(n, 256), ncolor), 256, 0, s>>>(m, w, prr, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj,
+                                                             m.ncell, cstride);
       break;
     }
     default:
@@ -268,9 +277,11 @@ int bca_ctx_destroy(bca_ctx* c) {
   for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
   free_work(c->w0);
   free_work(c->w1);
-  free_work(c->w5);
   if (c->norm_part) (void)hipFree(c->norm_part);
   if (c->norm_out) (void)hipFree(c->norm_out);
+  for (double* p : {c->blki, c->blkj, c->ghostT})
+    if (p) (void)hipFree(p);
+  if (c->wcol) (void)hipFree(c->wcol);
   delete c;
   return 0;
 }
@@ -413,7 +424,7 @@ int bca_bc_join(bca_ctx* c, double* a, const int prr[4], const int prd[4], const
   const Box rb{prr[0], prr[1], prr[2], prr[3]}, db{prd[0], prd[1], prd[2], prd[3]};
   const long plane = (long)ni_arr * nj_arr;
   k_join<double><<<grid1((long)cnt_i * cnt_j, 256), 256, 0, c->stream>>>(m, a, rb, db, tr[0], tr[1], em,
-                                                                          ni_arr, nj_arr, plane);
+                                                                          ni_arr, nj_arr, plane, 0);
   BCA_LAUNCHED();
   return 0;
 }
@@ -480,25 +491,73 @@ int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_p
   const Mesh& m = c->m;
   const int step = 2 * m.gh + 1;
   if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
-  int rc = ensure_work(c, c->w5, true);
-  if (rc) return rc;
+  int rc = 0;
+  if ((rc = ensure_work(c, c->w0, false))) return rc;
+  const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
+  const long ng = ghost_count(m);
+  const int ncol = step * step;
+  if ((rc = alloc_planes(c, &c->blki, (long)NOFF * 25 * nfi))) return rc;
+  if ((rc = alloc_planes(c, &c->blkj, (long)NOFF * 25 * nfj))) return rc;
+  if ((rc = alloc_planes(c, &c->ghostT, (long)ncol * ng * 25))) return rc;
+  if ((rc = alloc_planes(c, &c->wcol, (long)ncol * 5 * m.ncell))) return rc;
   const Metric gm = to_metric(g);
   const Phys ph = to_phys(p);
+  hipStream_t s = c->stream;
   const dim3 b(64, 4);
-  for (int l = 0; l < step; ++l)
-    for (int k = 0; k < step; ++k) {
-      k_seed_color<<<grid1(5 * m.ncell, 256), 256, 0, c->stream>>>(m, w, c->w5.w, l, k);
-      BCA_LAUNCHED();
-      for (int q = 0; q < nops; ++q)
-        if ((rc = launch_bc<Dual<5>>(c, c->w5.w, ops[q], g.nx, g.ny))) return rc;
-      if ((rc = run_pipeline<Dual<5>>(c, c->w5, c->w5.w, nullptr, gm, ph))) return rc;
-      {
-        Timed t(c, "k_balance_scatter<dual5>");
-        k_balance_scatter<<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w5.hn, jac, ia, ja, l, k,
-                                                                     coefdiag, withjn, write_index);
-      }
-      BCA_LAUNCHED();
-    }
+  // primal primitives and gradients of the state (ghost frame as given)
+  k_prim<double><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, c->w0.prim);
+  BCA_LAUNCHED();
+  k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, gm, c->w0.prim, c->w0.grad);
+  BCA_LAUNCHED();
+  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
+  BCA_LAUNCHED();
+  // phase 1: face-local blocks
+  BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, 3, m.jm};
+  {
+    Timed t(c, "k_iface_blocks");
+    launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
+  }
+  ai.j0 = 1; ai.j1 = 2;
+  launch_iface_blocks_o2(grid2(m.im + 1, 2, b), b, s, ai);
+  BCA_LAUNCHED();
+  BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, 4, m.jm + 1};
+  {
+    Timed t(c, "k_jface_blocks");
+    launch_jface_blocks_c(grid2(m.im, m.jm - 2, b), b, s, aj);
+  }
+  aj.j0 = aj.j1 = 3;
+  launch_jface_blocks_r3(grid2(m.im, 1, b), b, s, aj);
+  aj.j0 = aj.j1 = 2;
+  launch_jface_blocks_r2(grid2(m.im, 1, b), b, s, aj);
+  aj.j0 = aj.j1 = 1;
+  launch_jface_blocks_wall(grid2(m.im, 1, b), b, s, aj);
+  BCA_LAUNCHED();
+  // ghost tangents of every color through the linearised BC sequence, all
+  // 49 colors per launch
+  {
+    Timed t(c, "ghost_tangents");
+    const long cs = 5 * m.ncell;
+    k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), ncol), 256, 0, s>>>(m, w, c->wcol, cs);
+    BCA_LAUNCHED();
+    for (int q = 0; q < nops; ++q)
+      if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, ncol, cs))) return rc;
+    k_ghost_tangents<<<dim3(grid1(ng, 256), ncol), 256, 0, s>>>(m, c->wcol, cs, c->ghostT);
+  }
+  BCA_LAUNCHED();
+  // phase 2: rows
+  {
+    Timed t(c, "k_assemble_colored");
+    dim3 gr = grid2(m.im, m.jm, b);
+    gr.z = step;
+    k_assemble_colored<<<gr, b, 0, s>>>(m, c->blki, c->blkj, coefdiag, withjn, jac, ia, ja, write_index);
+  }
+  BCA_LAUNCHED();
+  {
+    Timed t(c, "k_assemble_ghosts");
+    k_assemble_ghosts<<<dim3(grid1(near_count(m), 256), ncol), 256, 0, s>>>(m, c->blki, c->blkj, c->ghostT,
+                                                                             withjn, jac);
+  }
+  BCA_LAUNCHED();
   return 0;
 }
 

@@ -222,8 +222,10 @@ __device__ inline auto diss_scalars(const A& a, int d) {
   using RS = prom_t<R, RG>;
   RS eps2(0.0);
   if (a.ph.k2 != 0.0) {
-    auto pm2 = a.template P<-2 * DX, -2 * DY>(P_P), pm1 = a.template P<-DX, -DY>(P_P);
-    auto p0 = a.template P<0, 0>(P_P), pp1 = a.template P<DX, DY>(P_P);
+    auto pm2 = a.template P<-2 * DX, -2 * DY>(P_P);
+    auto pm1 = a.template P<-DX, -DY>(P_P);
+    auto p0 = a.template P<0, 0>(P_P);
+    auto pp1 = a.template P<DX, DY>(P_P);
     auto ks1 = dabs(pm1 - 2.0 * p0 + pp1) / dabs(pm1 + 2.0 * p0 + pp1);
     auto ks2 = dabs(pm2 - 2.0 * pm1 + p0) / dabs(pm2 + 2.0 * pm1 + p0);
     auto divr = a.template G<0, 0>(G_UX) + a.template G<0, 0>(G_VY);

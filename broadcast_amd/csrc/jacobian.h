@@ -1,0 +1,286 @@
+// Colored sparse-Jacobian assembly from face-local Jacobian blocks.
+//
+// The reference builds -dR/dw column-group by column-group: 245 one-hot seeds
+// (5 variables x 7 x 7 spatial colors), each a full tangent evaluation of BCs
+// + residual, scattered into COO (BROADCAST_npz.py:1068-1127,
+// misc/ComputeJacobian.f90:357-374, 503-570).  Here the same numbers come from
+// two passes that never evaluate a dense tangent:
+//
+//  phase 1  (k_iface_blocks / k_jface_blocks) every face computes
+//           B(face, o) = d hn(face) / d w(face + o) for the 24 stencil offsets
+//           o of its family by compile-time sparse forward mode (SeededAcc),
+//           ghost cells included as independent inputs, one plane per offset;
+//  ghosts   (k_seed_frames + linearised BCs + k_ghost_tangents, all 49 colors
+//           per launch) the tangent the BC sequence gives every ghost cell,
+//           T(color, g), computed on the boundary frame only;
+//  phase 2  (k_assemble_colored, then k_assemble_ghosts on the rows near the
+//           mesh edge) for every row cell and color, the tangent residual of
+//           that color -- the sum over the cell's 4 faces of the block of the
+//           color's interior cell in the face stencil, plus B(face, ghost)
+//           T(color, ghost) -- written into the reference's COO slots with its
+//           window rules.
+//
+// For every seed this is exactly the reference's JVP (the tangent of a face is
+// the sum over its active inputs), so the COO values agree to rounding.
+#pragma once
+#include "blockskern.h"
+#include "misc.h"
+
+namespace bca {
+
+// ------------------------------------------------------------------ ghosts
+// Storage coordinates (a, b) of the t-th cell of the frame of width W around
+// the block (all cells within W of the array edge): rows b < W, rows
+// b >= nj - W, then the left/right W columns of the rows in between.  A block
+// too small for a frame of that width is taken whole, row by row.
+__host__ __device__ inline bool frame_whole(const Mesh& m, int W) { return 2 * W >= m.ni || 2 * W >= m.nj; }
+__host__ __device__ inline long frame_count(const Mesh& m, int W) {
+  if (frame_whole(m, W)) return m.ncell;
+  return 2L * W * m.ni + 2L * W * (m.nj - 2 * W);
+}
+__device__ inline void frame_cell(const Mesh& m, int W, long t, int& a, int& b) {
+  if (frame_whole(m, W) || t < (long)W * m.ni) { b = (int)(t / m.ni); a = (int)(t % m.ni); return; }
+  t -= (long)W * m.ni;
+  if (t < (long)W * m.ni) { b = m.nj - W + (int)(t / m.ni); a = (int)(t % m.ni); return; }
+  t -= (long)W * m.ni;
+  b = W + (int)(t / (2 * W));
+  const int s = (int)(t % (2 * W));
+  a = s < W ? s : m.ni - 2 * W + s;
+}
+
+// Seeded states of all 49 colors at once (blockIdx.y = color), written only
+// on the frame the boundary conditions read or write: the ghost frame plus 4
+// interior layers.  State values from w, unit tangent d[m] on variable m of
+// the cells seeded by the color, zero elsewhere.
+constexpr int BC_FRAME = 7;
+__global__ void k_seed_frames(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ ws, long cstride) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= frame_count(m, BC_FRAME)) return;
+  const int color = blockIdx.y;
+  const int step = 2 * m.gh + 1, l = color / step, k = color % step;
+  int a, b;
+  frame_cell(m, BC_FRAME, t, a, b);
+  const int i = a - m.gh + 1, j = b - m.gh + 1;
+  const bool on = i >= l + 1 && i <= m.im && j >= k + 1 && j <= m.jm && (i - l - 1) % step == 0 &&
+                  (j - k - 1) % step == 0;
+  const long c = a + (long)m.ni * b;
+  Dual<5>* out = ws + (long)color * cstride;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    Dual<5> x(w[c + e * m.ncell]);
+    if (on) x.d[e] = 1.0;
+    out[c + e * m.ncell] = x;
+  }
+}
+
+// T(color, g)[q][m]: tangent of ghost cell g, variable q, under seed (m, color)
+__global__ void k_ghost_tangents(Mesh m, const Dual<5>* __restrict__ ws, long cstride, double* __restrict__ T) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ng = ghost_count(m);
+  if (g >= ng) return;
+  const int color = blockIdx.y;
+  int a, b;
+  frame_cell(m, m.gh, g, a, b);  // the ghost frame, in ghost_index order
+  const long c = a + (long)m.ni * b;
+  const Dual<5>* src = ws + (long)color * cstride;
+  double* out = T + ((long)color * ng + g) * 25;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const Dual<5> x = src[c + q * m.ncell];
+#pragma unroll
+    for (int mm = 0; mm < 5; ++mm) out[q * 5 + mm] = x.d[mm];
+  }
+}
+
+// ------------------------------------------------------------------ phase 2
+__device__ inline bool interior(const Mesh& m, int i, int j) {
+  return i >= 1 && i <= m.im && j >= 1 && j <= m.jm;
+}
+
+// Row offset dj (relative to row j) of the cell of seed color k that lies in
+// the row's stencil: the residue of k - (j - 1) taken in [-3, 3], moved by one
+// period when that cell is outside the mesh (the near-wall Euler flux of the
+// face j = 2 reaches the cell j = 5 from the row j = 1).
+__device__ inline int color_offset(int k, int j, int n, int gh) {
+  const int step = 2 * gh + 1;
+  int d = ((k - (j - 1)) % step + step) % step;
+  if (d > gh) d -= step;
+  if (j + d < 1) d += step;
+  else if (j + d > n) d -= step;
+  return d;
+}
+
+// Interior part.  Grid (x: i, y: j, z: color row k), one thread per (row cell,
+// k).  Within a wave j and k are uniform, so is the column row offset dj; the
+// row's blocks for the 7 column offsets di = -3..3 are sums of the blocks of
+// its 4 faces read at wave-uniform offset planes (coalesced), and color l of
+// the lane picks di = residue of l - (i - 1) by a select chain.  A stencil
+// holds at most one interior cell per color (its widest extent, 8 cells,
+// occurs only on the first/last face where one end is a ghost), so this is the
+// whole interior contribution of each seed.  Values -dR/dw (+ coefdiag on the
+// diagonal) go to the reference's COO slots (misc/ComputeJacobian.f90:503-570).
+__global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, const double* __restrict__ bi,
+                                                          const double* __restrict__ bj,
+                                                          const double* __restrict__ coefdiag, int withjn,
+                                                          double* __restrict__ jac, int* __restrict__ ia,
+                                                          int* __restrict__ ja, int write_index) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > m.im || j > m.jm) return;
+  const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
+  const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
+  const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
+  const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
+  const int dj = color_offset(k, j, jm, gh);
+  const bool jin = j + dj >= 1 && j + dj <= jm;
+  // wave-uniform plane offsets of the 4 faces for each di (-1: no block)
+  long pi0[7], pi1[7], pj0[7], pj1[7];
+#pragma unroll
+  for (int D = 0; D < 7; ++D) {
+    const int di = D - 3;
+    int o = ioff_index(di, dj);
+    pi0[D] = (jin && o >= 0 && ioff_live(m, o, i)) ? (long)o * 25 * nfi + fi0 : -1;
+    o = ioff_index(di - 1, dj);
+    pi1[D] = (jin && o >= 0 && ioff_live(m, o, i + 1)) ? (long)o * 25 * nfi + fi1 : -1;
+    o = joff_index(di, dj);
+    pj0[D] = (jin && o >= 0 && joff_live(m, o, j)) ? (long)o * 25 * nfj + fj0 : -1;
+    o = joff_index(di, dj - 1);
+    pj1[D] = (jin && o >= 0 && joff_live(m, o, j + 1)) ? (long)o * 25 * nfj + fj1 : -1;
+    if (i + di < 1 || i + di > im) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
+  }
+  // seed columns of the 7 colors (l, k) of this row
+  int col[7];
+  bool ok[7];
+  const int rot = (i - 1) % step;
+#pragma unroll
+  for (int l = 0; l < 7; ++l) {
+    int vali, valj;
+    ok[l] = seed_column(i, j, l, k, gh, im, jm, withjn != 0, vali, valj);
+    col[l] = ok[l] ? valj * 5 + vali * jm * 5 : 0;
+  }
+  const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
+  const long plane = (long)im * jm;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5;
+  const int dummy = withjn ? 5 * im * jm - 2 : 5 * im * jm - 1;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
+#pragma unroll
+    for (int mm = 0; mm < 5; ++mm) {
+      const int q = e * 5 + mm;
+      double v[7];
+#pragma unroll
+      for (int D = 0; D < 7; ++D) {
+        double acc = 0.0;
+        // jac = -dR/dw with R(i,j) = hi(i) - hi(i+1) + hj(j) - hj(j+1)
+        if (pi0[D] >= 0) acc -= bi[pi0[D] + q * nfi];
+        if (pi1[D] >= 0) acc += bi[pi1[D] + q * nfi];
+        if (pj0[D] >= 0) acc -= bj[pj0[D] + q * nfj];
+        if (pj1[D] >= 0) acc += bj[pj1[D] + q * nfj];
+        v[D] = acc;
+      }
+#pragma unroll
+      for (int l = 0; l < 7; ++l) {
+        // column offset di of color l: residue of l - (i-1) in [-3, 3]
+        int r = l - rot;
+        r += r < -3 ? 7 : 0;
+        r -= r > 3 ? 7 : 0;
+        double x = v[0];
+#pragma unroll
+        for (int D = 1; D < 7; ++D) x = (r == D - 3) ? v[D] : x;
+        const long cur = base + (long)e * plane + (long)l * plane * 5 * step + (long)mm * plane * 5 * step * step;
+        const int c = ok[l] ? col[l] + mm : 0;
+        double val = ok[l] ? x : 0.0;
+        if (ok[l] && coefdiag && row == c) val += cd;
+        jac[cur] = val;
+        if (write_index) {
+          ia[cur] = ok[l] ? row : dummy;
+          ja[cur] = c;
+        }
+      }
+    }
+  }
+}
+
+// Ghost part, rows within NEAR cells of the mesh edge only: for each seed color
+// c, sum over the row's faces and the ghost cells g of their stencils of
+// B(face, g) T(c, g), added to the values k_assemble_colored wrote.
+constexpr int NEAR = 5;
+__host__ __device__ inline bool near_whole(const Mesh& m) { return 2 * NEAR >= m.im || 2 * NEAR >= m.jm; }
+__host__ __device__ inline long near_count(const Mesh& m) {
+  if (near_whole(m)) return (long)m.im * m.jm;
+  return 2L * NEAR * m.im + 2L * NEAR * (m.jm - 2 * NEAR);
+}
+__device__ inline void near_cell(const Mesh& m, long t, int& i, int& j) {
+  const int W = NEAR;
+  if (near_whole(m) || t < (long)W * m.im) { j = 1 + (int)(t / m.im); i = 1 + (int)(t % m.im); return; }
+  t -= (long)W * m.im;
+  if (t < (long)W * m.im) { j = m.jm - W + 1 + (int)(t / m.im); i = 1 + (int)(t % m.im); return; }
+  t -= (long)W * m.im;
+  j = W + 1 + (int)(t / (2 * W));
+  const int s = (int)(t % (2 * W));
+  i = 1 + (s < W ? s : m.im - 2 * W + s);
+}
+
+template <bool JFAM>
+__device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, long f, int fi, int fj, double sg,
+                                  const double* Tc, long ng, double acc[25]) {
+  for (int o = 0; o < NOFF; ++o) {
+    if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
+    const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
+    const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
+    if (interior(m, ci, cj)) continue;
+    if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
+    const double* tg = Tc + ghost_index(m, ci + m.gh - 1, cj + m.gh - 1) * 25;
+    const double* b = blk + (long)o * 25 * nf + f;
+    double bb[25];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) bb[q] = b[q * nf];
+#pragma unroll
+    for (int e = 0; e < 5; ++e)
+#pragma unroll
+      for (int mm = 0; mm < 5; ++mm) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s += bb[e * 5 + q] * tg[q * 5 + mm];
+        acc[e * 5 + mm] += sg * s;
+      }
+  }
+}
+
+// Grid (x: near rows, y: color 0..48).
+__global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, const double* __restrict__ bi,
+                                                         const double* __restrict__ bj,
+                                                         const double* __restrict__ T, int withjn,
+                                                         double* __restrict__ jac) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= near_count(m)) return;
+  const int color = blockIdx.y;
+  const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
+  const int l = color / step, k = color % step;
+  int i, j;
+  near_cell(m, t, i, j);
+  int vali, valj;
+  if (!seed_column(i, j, l, k, gh, im, jm, withjn != 0, vali, valj)) return;
+  const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
+  const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
+  const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
+  const long ng = ghost_count(m);
+  const double* Tc = T + (long)color * ng * 25;
+  double acc[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) acc[q] = 0.0;
+  ghost_face<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, ng, acc);
+  ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, ng, acc);
+  ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, ng, acc);
+  ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, ng, acc);
+  const long plane = (long)im * jm;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step;
+#pragma unroll
+  for (int e = 0; e < 5; ++e)
+#pragma unroll
+    for (int mm = 0; mm < 5; ++mm) jac[base + (long)e * plane + (long)mm * plane * 5 * step * step] += acc[e * 5 + mm];
+}
+
+}  // namespace bca

@@ -289,57 +289,4 @@ __global__ void k_unpack_interior(Mesh m, const Dual<N>* __restrict__ x, double*
   }
 }
 
-// State with the 5 one-hot seeds of color (l, k) as tangent directions m = 0..4.
-__global__ void k_seed_color(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ out, int l,
-                             int k) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= 5 * m.ncell) return;
-  const int e = (int)(c / m.ncell);
-  const long cc = c % m.ncell;
-  const int i = (int)(cc % m.ni) - m.gh + 1, j = (int)(cc / m.ni) - m.gh + 1;
-  const int step = 2 * m.gh + 1;
-  Dual<5> x(w[c]);
-  const bool on = i >= l + 1 && i <= m.im && j >= k + 1 && j <= m.jm && (i - l - 1) % step == 0 &&
-                  (j - k - 1) % step == 0;
-  if (on) x.d[e] = 1.0;
-  out[c] = x;
-}
-
-// Balance of one color's 5-direction tangent residual, scattered straight into
-// the reference's COO layout for seeds (m = 0..4, l, k).
-__global__ void k_balance_scatter(Mesh m, const Dual<5>* __restrict__ hn, double* __restrict__ jac,
-                                  int* __restrict__ ia, int* __restrict__ ja, int l, int k,
-                                  const double* __restrict__ coefdiag, int withjn, int write_index) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > m.im || j > m.jm) return;
-  const int im = m.im, jm = m.jm, step = 2 * m.gh + 1;
-  int vali, valj;
-  const bool ok = seed_column(i, j, l, k, m.gh, im, jm, withjn != 0, vali, valj);
-  const int dummy = withjn ? 5 * im * jm - 2 : 5 * im * jm - 1;
-  const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
-#pragma unroll
-  for (int e = 0; e < 5; ++e) {
-    const Dual<5> r = balance_at(m, hn, i, j, e);
-    const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-#pragma unroll
-    for (int mv = 0; mv < 5; ++mv) {
-      const long cur = (long)(i - 1) + (long)(j - 1) * im + (long)e * im * jm + (long)k * im * jm * 5 +
-                       (long)l * im * jm * 5 * step + (long)mv * im * jm * 5 * step * step;
-      double v = 0.0;
-      int col = 0;
-      if (ok) {
-        col = mv + valj * 5 + vali * jm * 5;
-        v = -r.d[mv];
-        if (coefdiag && row == col) v = cd - r.d[mv];
-      }
-      jac[cur] = v;
-      if (write_index) {
-        ia[cur] = ok ? row : dummy;
-        ja[cur] = col;
-      }
-    }
-  }
-}
-
 }  // namespace bca
