@@ -42,6 +42,7 @@ _SIGS = {
     "bca_volf": [_dp] * 3,
     "bca_residual_dnc5": [_dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_jvp_dnc5": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_vjp_dnc5": [_dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_bc_inlet": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, C.c_double],
     "bca_bc_noref": [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_double],
     "bca_bc_extrap2": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), C.c_int],
@@ -52,9 +53,14 @@ _SIGS = {
     "bca_norml2": [_dp, _dp, C.POINTER(C.c_double)],
     "bca_seed": [_dp, _dp, C.c_int, C.c_int, C.c_int],
     "bca_scatter_jv": [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_long],
+    "bca_seed_partial": [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "bca_scatter_jv_2zones": [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int,
+                              C.c_long],
     "bca_set_bndbl": [_dp, _dp, _dp, _dp],
     "bca_jacobian_colored": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
                              C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
+    "bca_jacobian_colored_2zones": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
+                                    C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
 }
 
 # Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).

@@ -124,34 +124,40 @@ def geometric_nodes(n: int, d1: float, ratio: float) -> np.ndarray:
 
 
 def _blasius_table(eta_max: float = 15.0, n: int = 15001):
-    """Blasius f''' + f f''/2 = 0, f(0)=f'(0)=0, f'(inf)=1, by RK4 shooting."""
+    """Blasius f''' + f f''/2 = 0, f(0)=f'(0)=0, f'(inf)=1: RK4 shooting on
+    s = f''(0), secant iteration on f'(eta_max; s) - 1 (smooth in s)."""
+    h = eta_max / (n - 1)
 
-    def integrate(s):
-        h = eta_max / (n - 1)
-        y = np.array([0.0, 0.0, s])
-        out = np.empty((n, 3))
-        out[0] = y
-
-        def rhs(v):
-            return np.array([v[1], v[2], -0.5 * v[0] * v[2]])
-
+    def integrate(s, keep=False):
+        a, b, c = 0.0, 0.0, s
+        out = np.empty((n, 3)) if keep else None
+        if keep:
+            out[0] = (a, b, c)
         for k in range(1, n):
-            k1 = rhs(y)
-            k2 = rhs(y + 0.5 * h * k1)
-            k3 = rhs(y + 0.5 * h * k2)
-            k4 = rhs(y + h * k3)
-            y = y + h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
-            out[k] = y
-        return out
+            k1a, k1b, k1c = b, c, -0.5 * a * c
+            a2, b2, c2 = a + 0.5 * h * k1a, b + 0.5 * h * k1b, c + 0.5 * h * k1c
+            k2a, k2b, k2c = b2, c2, -0.5 * a2 * c2
+            a3, b3, c3 = a + 0.5 * h * k2a, b + 0.5 * h * k2b, c + 0.5 * h * k2c
+            k3a, k3b, k3c = b3, c3, -0.5 * a3 * c3
+            a4, b4, c4 = a + h * k3a, b + h * k3b, c + h * k3c
+            k4a, k4b, k4c = b4, c4, -0.5 * a4 * c4
+            a += h / 6.0 * (k1a + 2 * k2a + 2 * k3a + k4a)
+            b += h / 6.0 * (k1b + 2 * k2b + 2 * k3b + k4b)
+            c += h / 6.0 * (k1c + 2 * k2c + 2 * k3c + k4c)
+            if keep:
+                out[k] = (a, b, c)
+        return out if keep else b
 
-    lo, hi = 0.2, 0.5
-    for _ in range(60):
-        mid = 0.5 * (lo + hi)
-        if integrate(mid)[-1, 1] > 1.0:
-            hi = mid
-        else:
-            lo = mid
-    sol = integrate(0.5 * (lo + hi))
+    s0, s1 = 0.3, 0.35
+    f0, f1 = integrate(s0) - 1.0, integrate(s1) - 1.0
+    for _ in range(50):
+        if f1 == f0:
+            break
+        s0, f0, s1 = s1, f1, s1 - f1 * (s1 - s0) / (f1 - f0)
+        f1 = integrate(s1) - 1.0
+        if abs(s1 - s0) <= 1e-15 * abs(s1):
+            break
+    sol = integrate(s1, keep=True)
     eta = np.linspace(0.0, eta_max, n)
     return eta, sol[:, 0], sol[:, 1]
 

@@ -13,7 +13,8 @@ struct BlockArgs {
   const double* w;
   const double* prim;
   const double* grad;
-  double* blk;
+  double* out;          // blocks (25 planes per offset) or VJP contributions (5)
+  const double* resb;   // residual adjoint (VJP only)
   int j0, j1;
 };
 
@@ -23,5 +24,11 @@ void launch_jface_blocks_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_iface_vjp_o4(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_iface_vjp_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 
 }  // namespace bca

@@ -1,7 +1,10 @@
-// Face-local Jacobian block kernels (phase 1 of the colored assembly):
-// B(face, o) = d hn(face) / d w(face + o) for the 24 stencil offsets o of the
-// face family, by compile-time sparse forward mode (SeededAcc).  Templates
-// only: each variant is instantiated in its own object (blocks.hip).
+// Face-local Jacobian block kernels: B(face, o) = d hn(face) / d w(face + o)
+// for the 24 stencil offsets o of the face family, by compile-time sparse
+// forward mode (SeededAcc).  Two outputs of the same passes:
+//   MODE_BLOCKS  the blocks themselves (phase 1 of the colored assembly);
+//   MODE_VJP     hnb(face)^T B(face, o), 5 values per offset, with hnb the
+//                face adjoint of residub (the VJP, gathered per cell later).
+// Templates only: each variant is instantiated in its own object (blocks.hip).
 #pragma once
 #include "faces.h"
 
@@ -60,6 +63,8 @@ __host__ __device__ inline long ghost_count(const Mesh& m) {
 }
 
 // ------------------------------------------------------------------ phase 1
+enum { MODE_BLOCKS = 0, MODE_VJP = 1 };
+
 template <int SI, int SJ>
 __device__ inline SeededAcc<SI, SJ> seeded(const Mesh& m, const Phys& ph, const Metric& g,
                                            const double* w, const double* prim, const double* grad,
@@ -71,14 +76,47 @@ __device__ inline SeededAcc<SI, SJ> seeded(const Mesh& m, const Phys& ph, const 
   return a;
 }
 
-// B(face f, offset o) goes to plane o: out[(o * 25 + e * 5 + q) * nf + f], so a
-// wave (consecutive faces, one offset) stores 25 coalesced rows.
-__device__ inline void store_block(double* blk, long nf, long f, int o, const Dual<5> hn[5]) {
-  double* out = blk + (long)o * 25 * nf + f;
+// Output of the seeded pass of offset o at face f.  MODE_BLOCKS: B(face, o)
+// in plane o, out[(o * 25 + e * 5 + q) * nf + f]; MODE_VJP: sum_e hb[e]
+// B(face, o)[e][q] in out[(o * 5 + q) * nf + f].  Either way a wave
+// (consecutive faces, one offset) stores coalesced rows.
+template <int MODE>
+struct FaceSink {
+  double* out;
+  long nf, f;
+  double hb[5];
+  __device__ void put(int o, const Dual<5> hn[5]) const {
+    if constexpr (MODE == MODE_BLOCKS) {
+      double* p = out + (long)o * 25 * nf + f;
 #pragma unroll
-  for (int e = 0; e < 5; ++e)
+      for (int e = 0; e < 5; ++e)
 #pragma unroll
-    for (int q = 0; q < 5; ++q) out[(e * 5 + q) * nf] = hn[e].d[q];
+        for (int q = 0; q < 5; ++q) p[(e * 5 + q) * nf] = hn[e].d[q];
+    } else {
+      double* p = out + (long)o * 5 * nf + f;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double t = 0.0;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) t += hb[e] * hn[e].d[q];
+        p[q * nf] = t;
+      }
+    }
+  }
+};
+
+// Face adjoints of the balance R(i,j) = hn(i) - hn(i+1) + hn(j) - hn(j+1) over
+// the interior rows (srcfv/adjoint/flux_num_dnc5_b.f90:800-829).
+__device__ inline void face_adjoint(const Mesh& m, const double* resb, int i, int j, int di, int dj,
+                                    double hb[5]) {
+  const bool lo = (i - di) >= 1 && (j - dj) >= 1, hi = i <= m.im && j <= m.jm;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    double v = 0.0;
+    if (hi) v += resb[cidx(m, i, j) + (long)e * m.ncell];
+    if (lo) v -= resb[cidx(m, i - di, j - dj) + (long)e * m.ncell];
+    hb[e] = v;
+  }
 }
 
 // i-faces (i, j), i in [1, im+1], rows j in [j0, j1]; O4 selects the
@@ -86,16 +124,18 @@ __device__ inline void store_block(double* blk, long nf, long f, int o, const Du
 // The two extra offsets (3, 0), (-4, 0) only exist for the faces i = 1 and
 // i = im+1 (extrapolated ghost gradients); elsewhere nothing is stored for
 // them and the assembly does not read them.
-template <bool O4>
+template <bool O4, int MODE>
 __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g, const double* __restrict__ w,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
-                                                      double* __restrict__ blk, int j0, int j1) {
+                                                      double* __restrict__ out, const double* __restrict__ resb,
+                                                      int j0, int j1) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > m.im + 1 || j > j1) return;
   const long nf = (long)(m.im + 1) * m.jm;
-  const long f = (i - 1) + (long)(m.im + 1) * (j - 1);
+  FaceSink<MODE> sink{out, nf, (i - 1) + (long)(m.im + 1) * (j - 1), {}};
+  if constexpr (MODE == MODE_VJP) face_adjoint(m, resb, i, j, 1, 0, sink.hb);
   sfor<0, NOFF>([&](auto O) {
     constexpr int o = decltype(O)::value;
     constexpr int SI = ioff_di(o), SJ = ioff_dj(o);
@@ -104,7 +144,7 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
     auto a = seeded<SI, SJ>(m, ph, g, w, prim, grad, i, j);
     Dual<5> hn[5];
     iface<O4>(a, hn);
-    store_block(blk, nf, f, o, hn);
+    sink.put(o, hn);
   });
 }
 
@@ -112,16 +152,18 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
 // (rows j >= 4), 3, 2 = off-centred near-wall rows, 1 = the wall face.
 // Extra offsets: (0, 3) only for row 2, (0, -4) only for row jm+1; the wall
 // face depends on (0, 0) and (0, 1) only.
-template <int ROW>
+template <int ROW, int MODE>
 __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g, const double* __restrict__ w,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
-                                                      double* __restrict__ blk, int j0, int j1) {
+                                                      double* __restrict__ out, const double* __restrict__ resb,
+                                                      int j0, int j1) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > m.im || j > j1) return;
   const long nf = (long)m.im * (m.jm + 1);
-  const long f = (i - 1) + (long)m.im * (j - 1);
+  FaceSink<MODE> sink{out, nf, (i - 1) + (long)m.im * (j - 1), {}};
+  if constexpr (MODE == MODE_VJP) face_adjoint(m, resb, i, j, 0, 1, sink.hb);
   sfor<0, NOFF>([&](auto O) {
     constexpr int o = decltype(O)::value;
     constexpr int SI = joff_di(o), SJ = joff_dj(o);
@@ -130,7 +172,7 @@ __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g,
       // the wall face does not depend on this cell: store an explicit zero
       // block where the assembly may look for one
       Dual<5> z[5];
-      store_block(blk, nf, f, o, z);
+      sink.put(o, z);
       return;
     } else {
       if (o == 23 && !(ROW == 0 && j == m.jm + 1)) return;
@@ -140,7 +182,7 @@ __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g,
       else if constexpr (ROW == 3) jface<3, true>(a, hn);
       else if constexpr (ROW == 2) jface<2, false>(a, hn);
       else jface_wall(a, hn);
-      store_block(blk, nf, f, o, hn);
+      sink.put(o, hn);
     }
   });
 }

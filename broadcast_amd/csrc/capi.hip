@@ -14,6 +14,7 @@
 #include "jacobian.h"
 #include "misc.h"
 #include "scheme.h"
+#include "vjp.h"
 
 using namespace bca;
 
@@ -69,8 +70,12 @@ struct bca_ctx {
   double* norm_out = nullptr;
   double* blki = nullptr;   // face-local Jacobian blocks, i-faces, one plane per offset
   double* blkj = nullptr;   // j-faces
-  double* ghostT = nullptr; // ghost tangents per color
-  Dual<5>* wcol = nullptr;  // seeded states of the 49 colors (frames only)
+  double* ghostT = nullptr; // ghost tangents per zone and color
+  Dual<5>* wcol = nullptr;  // seeded states of the zones x 49 colors (frames only)
+  unsigned char* ghostnz = nullptr;  // nonzero flags of the ghost tangents
+  int zone_cap = 0;         // number of zones the three arrays above hold
+  double* vjpi = nullptr;   // VJP contributions per face and offset, i-faces
+  double* vjpj = nullptr;   // j-faces
 };
 
 namespace {
@@ -111,6 +116,16 @@ int alloc_planes(bca_ctx* c, T** p, long count) {
   BCA_HIP(hipMalloc((void**)p, sizeof(T) * count));
   c->bytes += (long)sizeof(T) * count;
   return 0;
+}
+
+// (Re)allocates *p for count elements when cap < want (contents dropped).
+template <class T>
+int realloc_planes(bca_ctx* c, T** p, long count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  return alloc_planes(c, p, count);
 }
 
 template <class T>
@@ -279,9 +294,10 @@ int bca_ctx_destroy(bca_ctx* c) {
   free_work(c->w1);
   if (c->norm_part) (void)hipFree(c->norm_part);
   if (c->norm_out) (void)hipFree(c->norm_out);
-  for (double* p : {c->blki, c->blkj, c->ghostT})
+  for (double* p : {c->blki, c->blkj, c->ghostT, c->vjpi, c->vjpj})
     if (p) (void)hipFree(p);
   if (c->wcol) (void)hipFree(c->wcol);
+  if (c->ghostnz) (void)hipFree(c->ghostnz);
   delete c;
   return 0;
 }
@@ -475,6 +491,30 @@ int bca_scatter_jv(bca_ctx* c, double* jac, int* ia, int* ja, const double* resd
   return 0;
 }
 
+int bca_seed_partial(bca_ctx* c, double* wd, int mv, int l, int k, int istart, int iend, int jstart, int jend) {
+  if (!c || !wd) return fail(-1, "null argument");
+  const int step = 2 * c->m.gh + 1;
+  if (mv < 0 || mv > 4 || l < 0 || l >= step || k < 0 || k >= step) return fail(-1, "seed index out of range");
+  k_testvector_partial<<<grid1(5 * c->m.ncell, 256), 256, 0, c->stream>>>(c->m, wd, mv, l, k, istart + 1,
+                                                                           iend + 1, jstart + 1, jend + 1);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_scatter_jv_2zones(bca_ctx* c, double* jac, int* ia, int* ja, const double* resd, int mv, int l, int k,
+                          const double* coefdiag, double mini, int n, long nbentry) {
+  if (!c || !jac || !ia || !ja || !resd) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (mv < 0 || mv > 4 || l < 0 || l >= step || k < 0 || k >= step) return fail(-1, "seed index out of range");
+  if (n < 0 || n > 1) return fail(-1, "zone index must be 0 or 1");
+  if (nbentry < 2L * m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than 2*im*jm*(2gh+1)^2*25");
+  k_scatter_jv_zone<<<grid1(5L * m.im * m.jm, 256), 256, 0, c->stream>>>(m, jac, ia, ja, resd, mv, l, k, coefdiag,
+                                                                          mini, n);
+  BCA_LAUNCHED();
+  return 0;
+}
+
 int bca_set_bndbl(bca_ctx* c, const double* w, double* field, double* wbd) {
   if (!c || !w || !field || !wbd) return fail(-1, "null argument");
   const Mesh& m = c->m;
@@ -483,23 +523,34 @@ int bca_set_bndbl(bca_ctx* c, const double* w, double* field, double* wbd) {
   return 0;
 }
 
-int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
-                         const bca_bc_op* ops, int nops, const double* coefdiag, int withjn,
-                         double* jac, int* ia, int* ja, long nbentry, int write_index) {
+}  // extern "C"
+
+namespace {
+
+int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+                  const double* coefdiag, const Zones& z, double* jac, int* ia, int* ja, long nbentry,
+                  int write_index) {
   if (!c || !w || !p || !jac || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+  if (!g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null metrics");
   if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
   const Mesh& m = c->m;
   const int step = 2 * m.gh + 1;
-  if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
+  if (nbentry < (long)z.n * m.im * m.jm * step * step * 25)
+    return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
   int rc = 0;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
   const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
   const long ng = ghost_count(m);
-  const int ncol = step * step;
+  const int ncol = step * step, nslot = z.n * ncol;
   if ((rc = alloc_planes(c, &c->blki, (long)NOFF * 25 * nfi))) return rc;
   if ((rc = alloc_planes(c, &c->blkj, (long)NOFF * 25 * nfj))) return rc;
-  if ((rc = alloc_planes(c, &c->ghostT, (long)ncol * ng * 25))) return rc;
-  if ((rc = alloc_planes(c, &c->wcol, (long)ncol * 5 * m.ncell))) return rc;
+  if (c->zone_cap < z.n) {
+    c->bytes -= (long)c->zone_cap * ncol * (ng * 25 * (long)sizeof(double) + 5 * m.ncell * (long)sizeof(Dual<5>) + ng);
+    if ((rc = realloc_planes(c, &c->ghostT, (long)nslot * ng * 25))) return rc;
+    if ((rc = realloc_planes(c, &c->wcol, (long)nslot * 5 * m.ncell))) return rc;
+    if ((rc = realloc_planes(c, &c->ghostnz, (long)nslot * ng))) return rc;
+    c->zone_cap = z.n;
+  }
   const Metric gm = to_metric(g);
   const Phys ph = to_phys(p);
   hipStream_t s = c->stream;
@@ -511,8 +562,8 @@ int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_p
   BCA_LAUNCHED();
   k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
-  // phase 1: face-local blocks
-  BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, 3, m.jm};
+  // phase 1: face-local blocks (independent of the zones)
+  BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
   {
     Timed t(c, "k_iface_blocks");
     launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
@@ -520,7 +571,7 @@ int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_p
   ai.j0 = 1; ai.j1 = 2;
   launch_iface_blocks_o2(grid2(m.im + 1, 2, b), b, s, ai);
   BCA_LAUNCHED();
-  BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, 4, m.jm + 1};
+  BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, nullptr, 4, m.jm + 1};
   {
     Timed t(c, "k_jface_blocks");
     launch_jface_blocks_c(grid2(m.im, m.jm - 2, b), b, s, aj);
@@ -532,31 +583,102 @@ int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_p
   aj.j0 = aj.j1 = 1;
   launch_jface_blocks_wall(grid2(m.im, 1, b), b, s, aj);
   BCA_LAUNCHED();
-  // ghost tangents of every color through the linearised BC sequence, all
-  // 49 colors per launch
+  // ghost tangents of every zone and color through the linearised BC
+  // sequence, all of them per launch
   {
     Timed t(c, "ghost_tangents");
     const long cs = 5 * m.ncell;
-    k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), ncol), 256, 0, s>>>(m, w, c->wcol, cs);
+    k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), nslot), 256, 0, s>>>(m, z, w, c->wcol, cs);
     BCA_LAUNCHED();
     for (int q = 0; q < nops; ++q)
-      if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, ncol, cs))) return rc;
-    k_ghost_tangents<<<dim3(grid1(ng, 256), ncol), 256, 0, s>>>(m, c->wcol, cs, c->ghostT);
+      if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, nslot, cs))) return rc;
+    k_ghost_tangents<<<dim3(grid1(ng, 256), nslot), 256, 0, s>>>(m, c->wcol, cs, c->ghostT, c->ghostnz);
   }
   BCA_LAUNCHED();
   // phase 2: rows
   {
     Timed t(c, "k_assemble_colored");
     dim3 gr = grid2(m.im, m.jm, b);
-    gr.z = step;
-    k_assemble_colored<<<gr, b, 0, s>>>(m, c->blki, c->blkj, coefdiag, withjn, jac, ia, ja, write_index);
+    gr.z = step * z.n;
+    k_assemble_colored<<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
   }
   BCA_LAUNCHED();
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<<<dim3(grid1(near_count(m), 256), ncol), 256, 0, s>>>(m, c->blki, c->blkj, c->ghostT,
-                                                                             withjn, jac);
+    k_assemble_ghosts<<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
+                                                                              c->ghostnz, jac);
   }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
+                         const bca_bc_op* ops, int nops, const double* coefdiag, int withjn,
+                         double* jac, int* ia, int* ja, long nbentry, int write_index) {
+  if (!c) return fail(-1, "null argument");
+  const Zones z{1, {1, 0}, {c->m.im, 0}, withjn != 0};
+  return jacobian_impl(c, w, g, p, ops, nops, coefdiag, z, jac, ia, ja, nbentry, write_index);
+}
+
+int bca_jacobian_colored_2zones(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
+                                const bca_bc_op* ops, int nops, const double* coefdiag, int isplit,
+                                double* jac, int* ia, int* ja, long nbentry, int write_index) {
+  if (!c) return fail(-1, "null argument");
+  const int step = 2 * c->m.gh + 1;
+  if (isplit < step || c->m.im - isplit < step) return fail(-1, "each zone must be at least 2gh+1 cells wide");
+  const Zones z{2, {1, isplit + 1}, {isplit, c->m.im}, 1};
+  return jacobian_impl(c, w, g, p, ops, nops, coefdiag, z, jac, ia, ja, nbentry, write_index);
+}
+
+int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_metrics g, const bca_phys* p) {
+  if (!c || !residub || !wb || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  int rc = 0;
+  if ((rc = ensure_work(c, c->w0, false))) return rc;
+  const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
+  if ((rc = alloc_planes(c, &c->vjpi, (long)NOFF * 5 * nfi))) return rc;
+  if ((rc = alloc_planes(c, &c->vjpj, (long)NOFF * 5 * nfj))) return rc;
+  const Metric gm = to_metric(g);
+  const Phys ph = to_phys(p);
+  hipStream_t s = c->stream;
+  const dim3 b(64, 4);
+  // recomputed primal primitives and gradients (no tape)
+  k_prim<double><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, c->w0.prim);
+  BCA_LAUNCHED();
+  k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, gm, c->w0.prim, c->w0.grad);
+  BCA_LAUNCHED();
+  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
+  BCA_LAUNCHED();
+  BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 3, m.jm};
+  {
+    Timed t(c, "k_iface_vjp");
+    launch_iface_vjp_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
+  }
+  ai.j0 = 1; ai.j1 = 2;
+  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, s, ai);
+  BCA_LAUNCHED();
+  BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpj, residub, 4, m.jm + 1};
+  {
+    Timed t(c, "k_jface_vjp");
+    launch_jface_vjp_c(grid2(m.im, m.jm - 2, b), b, s, aj);
+  }
+  aj.j0 = aj.j1 = 3;
+  launch_jface_vjp_r3(grid2(m.im, 1, b), b, s, aj);
+  aj.j0 = aj.j1 = 2;
+  launch_jface_vjp_r2(grid2(m.im, 1, b), b, s, aj);
+  aj.j0 = aj.j1 = 1;
+  launch_jface_vjp_wall(grid2(m.im, 1, b), b, s, aj);
+  BCA_LAUNCHED();
+  {
+    Timed t(c, "k_vjp_gather");
+    k_vjp_gather<<<grid2(m.ni, m.nj, b), b, 0, s>>>(m, c->vjpi, c->vjpj, wb);
+  }
+  BCA_LAUNCHED();
+  k_zero_interior<<<grid2(m.im, m.jm, b), b, 0, s>>>(m, residub);
   BCA_LAUNCHED();
   return 0;
 }

@@ -28,6 +28,25 @@
 
 namespace bca {
 
+// Seed zones in i.  One zone spanning the mesh is the reference's plain
+// coloring (testvector + computejacobianfromjv[_relaxed][_withjn]); two zones
+// are the periodic two-zone coloring (testvector_partial +
+// computejacobianfromjv_relaxed_withjnandcheck, cylinder.py:895-935), each
+// zone filling its own block of the doubled COO.  lo/hi are 1-based inclusive.
+struct Zones {
+  int n;
+  int lo[2], hi[2];
+  int withjn;
+};
+
+// the row's seed column under the zone's window rules (empty-slot semantics)
+__device__ inline bool zone_column(const Mesh& m, const Zones& z, int zn, int i, int j, int l, int k, int& vali,
+                                   int& valj) {
+  if (z.n == 1) return seed_column(i, j, l, k, m.gh, m.im, m.jm, z.withjn != 0, vali, valj);
+  bool wrapped;
+  return seed_column_zone(i, j, l, k, m.gh, m.im, m.jm, zn, vali, valj, wrapped);
+}
+
 // ------------------------------------------------------------------ ghosts
 // Storage coordinates (a, b) of the t-th cell of the frame of width W around
 // the block (all cells within W of the array edge): rows b < W, rows
@@ -48,23 +67,26 @@ __device__ inline void frame_cell(const Mesh& m, int W, long t, int& a, int& b) 
   a = s < W ? s : m.ni - 2 * W + s;
 }
 
-// Seeded states of all 49 colors at once (blockIdx.y = color), written only
-// on the frame the boundary conditions read or write: the ghost frame plus 4
-// interior layers.  State values from w, unit tangent d[m] on variable m of
-// the cells seeded by the color, zero elsewhere.
+// Seeded states of all zones x 49 colors at once (blockIdx.y = zone * 49 +
+// color), written only on the frame the boundary conditions read or write:
+// the ghost frame plus 4 interior layers.  State values from w, unit tangent
+// d[m] on variable m of the cells the color seeds in the zone, zero elsewhere.
 constexpr int BC_FRAME = 7;
-__global__ void k_seed_frames(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ ws, long cstride) {
+__global__ void k_seed_frames(Mesh m, Zones z, const double* __restrict__ w, Dual<5>* __restrict__ ws,
+                              long cstride) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= frame_count(m, BC_FRAME)) return;
-  const int color = blockIdx.y;
-  const int step = 2 * m.gh + 1, l = color / step, k = color % step;
+  const int step = 2 * m.gh + 1, ncol = step * step;
+  const int color = blockIdx.y % ncol, zn = blockIdx.y / ncol;
+  const int l = color / step, k = color % step;
   int a, b;
   frame_cell(m, BC_FRAME, t, a, b);
   const int i = a - m.gh + 1, j = b - m.gh + 1;
-  const bool on = i >= l + 1 && i <= m.im && j >= k + 1 && j <= m.jm && (i - l - 1) % step == 0 &&
+  const int i0 = z.lo[zn];
+  const bool on = i >= i0 + l && i <= z.hi[zn] && j >= k + 1 && j <= m.jm && (i - i0 - l) % step == 0 &&
                   (j - k - 1) % step == 0;
   const long c = a + (long)m.ni * b;
-  Dual<5>* out = ws + (long)color * cstride;
+  Dual<5>* out = ws + (long)blockIdx.y * cstride;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     Dual<5> x(w[c + e * m.ncell]);
@@ -73,8 +95,12 @@ __global__ void k_seed_frames(Mesh m, const double* __restrict__ w, Dual<5>* __r
   }
 }
 
-// T(color, g)[q][m]: tangent of ghost cell g, variable q, under seed (m, color)
-__global__ void k_ghost_tangents(Mesh m, const Dual<5>* __restrict__ ws, long cstride, double* __restrict__ T) {
+// T(color, g)[q][m]: tangent of ghost cell g, variable q, under seed (m, color);
+// nz(color, g) = 1 where T(color, g) has a nonzero entry.  A ghost state
+// depends on a few interior cells only (mirror, extrapolation and join
+// sources), so most (color, ghost) pairs are zero and the fold skips them.
+__global__ void k_ghost_tangents(Mesh m, const Dual<5>* __restrict__ ws, long cstride, double* __restrict__ T,
+                                 unsigned char* __restrict__ nz) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ng = ghost_count(m);
   if (g >= ng) return;
@@ -84,12 +110,17 @@ __global__ void k_ghost_tangents(Mesh m, const Dual<5>* __restrict__ ws, long cs
   const long c = a + (long)m.ni * b;
   const Dual<5>* src = ws + (long)color * cstride;
   double* out = T + ((long)color * ng + g) * 25;
+  bool any = false;
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     const Dual<5> x = src[c + q * m.ncell];
 #pragma unroll
-    for (int mm = 0; mm < 5; ++mm) out[q * 5 + mm] = x.d[mm];
+    for (int mm = 0; mm < 5; ++mm) {
+      out[q * 5 + mm] = x.d[mm];
+      any = any || x.d[mm] != 0.0;
+    }
   }
+  nz[(long)color * ng + g] = any ? 1 : 0;
 }
 
 // ------------------------------------------------------------------ phase 2
@@ -119,14 +150,14 @@ __device__ inline int color_offset(int k, int j, int n, int gh) {
 // occurs only on the first/last face where one end is a ghost), so this is the
 // whole interior contribution of each seed.  Values -dR/dw (+ coefdiag on the
 // diagonal) go to the reference's COO slots (misc/ComputeJacobian.f90:503-570).
-__global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, const double* __restrict__ bi,
+__global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const double* __restrict__ bi,
                                                           const double* __restrict__ bj,
-                                                          const double* __restrict__ coefdiag, int withjn,
+                                                          const double* __restrict__ coefdiag,
                                                           double* __restrict__ jac, int* __restrict__ ia,
                                                           int* __restrict__ ja, int write_index) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  const int k = blockIdx.z % 7, zn = blockIdx.z / 7;
   if (i > m.im || j > m.jm) return;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
   const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
@@ -147,7 +178,7 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, const double* 
     pj0[D] = (jin && o >= 0 && joff_live(m, o, j)) ? (long)o * 25 * nfj + fj0 : -1;
     o = joff_index(di, dj - 1);
     pj1[D] = (jin && o >= 0 && joff_live(m, o, j + 1)) ? (long)o * 25 * nfj + fj1 : -1;
-    if (i + di < 1 || i + di > im) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
+    if (i + di < z.lo[zn] || i + di > z.hi[zn]) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
   }
   // seed columns of the 7 colors (l, k) of this row
   int col[7];
@@ -156,13 +187,13 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, const double* 
 #pragma unroll
   for (int l = 0; l < 7; ++l) {
     int vali, valj;
-    ok[l] = seed_column(i, j, l, k, gh, im, jm, withjn != 0, vali, valj);
+    ok[l] = zone_column(m, z, zn, i, j, l, k, vali, valj);
     col[l] = ok[l] ? valj * 5 + vali * jm * 5 : 0;
   }
   const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
   const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5;
-  const int dummy = withjn ? 5 * im * jm - 2 : 5 * im * jm - 1;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)zn * plane * 25 * step * step;
+  const int dummy = (z.withjn || z.n > 1) ? 5 * im * jm - 2 : 5 * im * jm - 1;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
@@ -225,14 +256,16 @@ __device__ inline void near_cell(const Mesh& m, long t, int& i, int& j) {
 
 template <bool JFAM>
 __device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, long f, int fi, int fj, double sg,
-                                  const double* Tc, long ng, double acc[25]) {
+                                  const double* Tc, const unsigned char* nzc, double acc[25]) {
   for (int o = 0; o < NOFF; ++o) {
     if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
     const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
     const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
     if (interior(m, ci, cj)) continue;
     if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
-    const double* tg = Tc + ghost_index(m, ci + m.gh - 1, cj + m.gh - 1) * 25;
+    const long g = ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
+    if (!nzc[g]) continue;
+    const double* tg = Tc + g * 25;
     const double* b = blk + (long)o * 25 * nf + f;
     double bb[25];
 #pragma unroll
@@ -249,34 +282,37 @@ __device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, lon
   }
 }
 
-// Grid (x: near rows, y: color 0..48).
-__global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, const double* __restrict__ bi,
+// Grid (x: near rows, y: zone * 49 + color).
+__global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const double* __restrict__ bi,
                                                          const double* __restrict__ bj,
-                                                         const double* __restrict__ T, int withjn,
+                                                         const double* __restrict__ T,
+                                                         const unsigned char* __restrict__ nz,
                                                          double* __restrict__ jac) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= near_count(m)) return;
-  const int color = blockIdx.y;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
+  const int color = blockIdx.y % (step * step), zn = blockIdx.y / (step * step);
   const int l = color / step, k = color % step;
   int i, j;
   near_cell(m, t, i, j);
   int vali, valj;
-  if (!seed_column(i, j, l, k, gh, im, jm, withjn != 0, vali, valj)) return;
+  if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) return;
   const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
   const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
   const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
   const long ng = ghost_count(m);
-  const double* Tc = T + (long)color * ng * 25;
+  const double* Tc = T + (long)blockIdx.y * ng * 25;
+  const unsigned char* nzc = nz + (long)blockIdx.y * ng;
   double acc[25];
 #pragma unroll
   for (int q = 0; q < 25; ++q) acc[q] = 0.0;
-  ghost_face<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, ng, acc);
-  ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, ng, acc);
-  ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, ng, acc);
-  ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, ng, acc);
+  ghost_face<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, nzc, acc);
+  ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, acc);
+  ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, acc);
+  ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, acc);
   const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step +
+                    (long)zn * plane * 25 * step * step;
 #pragma unroll
   for (int e = 0; e < 5; ++e)
 #pragma unroll

@@ -211,6 +211,86 @@ __host__ __device__ inline bool seed_column(int i, int j, int l, int k, int gh, 
   return true;
 }
 
+// Two-zone variant (computejacobianfromjv_relaxed_withjnandcheck,
+// misc/ComputeJacobian.f90:1095-1204) for an empty slot: zone 0 wraps the
+// columns at or past im-2gh to l, zone 1 those at or below 2gh to
+// im - mod(im, 2gh+1) + l (one period lower if that passes im-1).  `wrapped`
+// reports which rule applied (the check resets an occupied wrapped slot of
+// zone 0 and an occupied unwrapped slot of zone 1 to the dummy).
+__host__ __device__ inline bool seed_column_zone(int i, int j, int l, int k, int gh, int im, int jm, int n,
+                                                 int& vali, int& valj, bool& wrapped) {
+  const int step = 2 * gh + 1;
+  valj = (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
+  wrapped = false;
+  if (valj >= jm) return false;
+  if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
+  else vali = (i <= l - gh) ? l : (i - (l + 1) + gh) / step * step + l;
+  if (n == 0) {
+    wrapped = vali >= im - 2 * gh;
+    if (wrapped) vali = l;
+  } else {
+    wrapped = vali <= 2 * gh;
+    if (wrapped) {
+      vali = im - im % step + l;
+      if (vali > im - 1) vali -= step;
+    }
+  }
+  return true;
+}
+
+// testvector_partial (misc/ComputeJacobian.f90:1075-1092): seeds of (m, l, k)
+// restricted to the 1-based index box [i0, i1] x [j0, j1].
+__global__ void k_testvector_partial(Mesh m, double* __restrict__ wd, int mv, int l, int k, int i0, int i1,
+                                     int j0, int j1) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= 5 * m.ncell) return;
+  const int e = (int)(c / m.ncell);
+  const long cc = c % m.ncell;
+  const int i = (int)(cc % m.ni) - m.gh + 1, j = (int)(cc / m.ni) - m.gh + 1;
+  const int step = 2 * m.gh + 1;
+  const bool on = e == mv && i >= i0 + l && i <= i1 && j >= j0 + k && j <= j1 && (i - i0 - l) % step == 0 &&
+                  (j - j0 - k) % step == 0;
+  wd[c] = on ? 1.0 : 0.0;
+}
+
+// One JVP result of seed (m, l, k) of zone n into slot block n of the doubled
+// COO, writing only slots that hold |jac| < mini (the reference's check).
+__global__ void k_scatter_jv_zone(Mesh m, double* __restrict__ jac, int* __restrict__ ia, int* __restrict__ ja,
+                                  const double* __restrict__ resd, int mv, int l, int k,
+                                  const double* __restrict__ coefdiag, double mini, int n) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nint = (long)m.im * m.jm;
+  if (q >= 5 * nint) return;
+  const int e = (int)(q / nint);
+  const long r = q % nint;
+  const int i = 1 + (int)(r % m.im), j = 1 + (int)(r / m.im);
+  const int step = 2 * m.gh + 1, im = m.im, jm = m.jm;
+  const long cur = (long)(i - 1) + (long)(j - 1) * im + (long)e * im * jm + (long)k * im * jm * 5 +
+                   (long)l * im * jm * 5 * step + (long)mv * im * jm * 5 * step * step +
+                   (long)n * im * jm * 25 * step * step;
+  const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
+  const int dummy = 5 * im * jm - 2;
+  ia[cur] = row;
+  int vali, valj;
+  bool wrapped;
+  if (!seed_column_zone(i, j, l, k, m.gh, im, jm, n, vali, valj, wrapped)) {
+    ia[cur] = dummy;
+    ja[cur] = 0;
+    jac[cur] = 0.0;
+    return;
+  }
+  if (fabs(jac[cur]) < mini) {
+    const int col = mv + valj * 5 + vali * jm * 5;
+    ja[cur] = col;
+    const double rv = resd[cidx(m, i, j) + (long)e * m.ncell];
+    jac[cur] = (coefdiag && row == col) ? coefdiag[(i - 1) + (long)im * (j - 1)] - rv : -rv;
+  } else if ((n == 0) == wrapped) {
+    ia[cur] = dummy;
+    ja[cur] = 0;
+    jac[cur] = 0.0;
+  }
+}
+
 // One JVP result resd of seed (m, l, k) into the reference's COO layout.
 __global__ void k_scatter_jv(Mesh m, double* __restrict__ jac, int* __restrict__ ia,
                              int* __restrict__ ja, const double* __restrict__ resd, int mv, int l,

@@ -168,6 +168,27 @@ class Flow:
                                        C.byref(self.phys)), "bca_jvp_dnc5")
         return res, resd
 
+    def jvp_flux(self, w, wd, res=None, resd=None):
+        """dR/dw . wd of the flux routine alone (wd's ghost frame as given,
+        srcfv/tangent/flux_num_dnc5_d.f90:15)."""
+        res = self.new_state() if res is None else res
+        resd = self.new_state() if resd is None else resd
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_jvp_dnc5(h, _ptr(res), _ptr(resd), _ptr(w), _ptr(wd), self.metrics,
+                                       C.byref(self.phys)), "bca_jvp_dnc5")
+        return res, resd
+
+    def vjp(self, w, resb, wb=None, consume=False):
+        """wb = (dR/dw)^T resb over the whole array, ghost frame included
+        (srcfv/adjoint/flux_num_dnc5_b.f90:15).  The C call consumes resb's
+        interior like the reference; unless consume=True a copy is passed."""
+        wb = self.new_state() if wb is None else wb
+        rb = resb if consume else resb.clone()
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_vjp_dnc5(h, _ptr(rb), _ptr(wb), _ptr(w), self.metrics, C.byref(self.phys)),
+                   "bca_vjp_dnc5")
+        return wb
+
     def norms(self, res):
         out = (C.c_double * 10)()
         h = self.ctx.sync_stream()
@@ -176,9 +197,19 @@ class Flow:
         return o[:5], o[5:]
 
     @property
+    def two_zone_split(self):
+        """First i index (0-based) of the upper seed zone when the reference
+        colors this mesh in two zones (an O-mesh whose im is not a multiple of
+        2gh+1, cylinder.py:896-897), else None."""
+        step = 2 * self.gh + 1
+        if self.case.kind == "cyl" and self.im % step != 0:
+            return (self.im // 2) // step * step
+        return None
+
+    @property
     def nbentry(self):
         s = 2 * self.gh + 1
-        return self.im * self.jm * s * s * 25
+        return self.im * self.jm * s * s * 25 * (1 if self.two_zone_split is None else 2)
 
     def jacobian_buffers(self):
         t = self.t
@@ -187,13 +218,20 @@ class Flow:
                 t.zeros(n, dtype=t.int32, device="cuda"))
 
     def jacobian_colored(self, w, coefdiag=None, bufs=None, write_index=True):
-        """The reference's 245-seed assembly into its COO layout (jac, ia, ja)
+        """The reference's colored assembly into its COO layout (jac, ia, ja)
         on the device: -dR/dw (+ coefdiag on the diagonal); w must carry its
-        ghost frame (BCs applied)."""
+        ghost frame (BCs applied).  Meshes the reference colors in two zones
+        get the two-zone seeds and scatter over the doubled COO."""
         jac, ia, ja = self.jacobian_buffers() if bufs is None else bufs
         h = self.ctx.sync_stream()
-        _lib.check(self.L.bca_jacobian_colored(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops,
-                                               self.nops, _ptr(coefdiag), int(self.case.kind == "cyl"),
-                                               _ptr(jac), _ptr(ia), _ptr(ja), self.nbentry,
-                                               int(write_index)), "bca_jacobian_colored")
+        split = self.two_zone_split
+        if split is None:
+            rc = self.L.bca_jacobian_colored(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops,
+                                             _ptr(coefdiag), int(self.case.kind == "cyl"), _ptr(jac), _ptr(ia),
+                                             _ptr(ja), self.nbentry, int(write_index))
+        else:
+            rc = self.L.bca_jacobian_colored_2zones(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops,
+                                                    self.nops, _ptr(coefdiag), split, _ptr(jac), _ptr(ia),
+                                                    _ptr(ja), self.nbentry, int(write_index))
+        _lib.check(rc, "bca_jacobian_colored")
         return jac, ia, ja

@@ -15,6 +15,30 @@ def testvector(wd, m, l, k, gh, im, jm):
     dev.finish(st)
 
 
+def testvector_partial(wd, m, l, k, gh, im, jm, istart, iend, jstart, jend):
+    """misc/ComputeJacobian.f90:1075-1092 (0-based index box)"""
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(wd, out=(0,))
+    _lib.check(_lib.lib().bca_seed_partial(ctx.h, st[0].ptr, int(m), int(l), int(k), int(istart), int(iend),
+                                           int(jstart), int(jend)), "testvector_partial")
+    dev.finish(st)
+
+
+def computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, coefdiag, mini, n, im=None,
+                                                 jm=None, nbentry=None):
+    """misc/ComputeJacobian.f90:1095-1204, call form of cylinder.py:939 (im, jm
+    inferred from coefdiag)"""
+    im = im if im is not None else coefdiag.shape[0]
+    jm = jm if jm is not None else coefdiag.shape[1]
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(jac, ia, ja, resd, coefdiag, out=(0, 1, 2))
+    nb = jac.shape[0] if hasattr(jac, "shape") else len(jac)
+    _lib.check(_lib.lib().bca_scatter_jv_2zones(ctx.h, st[0].ptr, st[1].ptr, st[2].ptr, st[3].ptr, int(m), int(l),
+                                                int(k), st[4].ptr, float(mini), int(n), int(nb)),
+               "computejacobianfromjv_relaxed_withjnandcheck")
+    dev.finish(st)
+
+
 def _scatter(jac, ia, ja, resd, m, l, k, gh, im, jm, coefdiag, withjn):
     ctx = dev.context(im, jm, gh)
     st = dev.stage(jac, ia, ja, resd, coefdiag, out=(0, 1, 2))

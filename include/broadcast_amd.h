@@ -97,6 +97,15 @@ int bca_residual_dnc5(bca_ctx *ctx, double *residu, const double *w, bca_metrics
 int bca_jvp_dnc5(bca_ctx *ctx, double *residu, double *residud, const double *w,
                  const double *wd, bca_metrics g, const bca_phys *phys);
 
+/* flux_num_dnc5_2d_b(residu,residub,w,wb,...)
+ * replaces srcfv/adjoint/flux_num_dnc5_b.f90:15-6676: wb = (dR/dw)^T residub,
+ * with w's ghost frame as independent inputs (no BC adjoint), over the whole
+ * array including the ghost frame; wb is overwritten, residub is consumed
+ * (its interior is zeroed, as the reference does), residu is not touched.
+ * The primal is recomputed; nothing is stored between calls. */
+int bca_vjp_dnc5(bca_ctx *ctx, double *residub, double *wb, const double *w, bca_metrics g,
+                 const bca_phys *phys);
+
 /* ---- f_bnd (wd == NULL) and f_lin (wd != NULL: linearised BC, w and wd
  * both updated, as the TAPENADE _d routines) -------------------------------- */
 /* bc_supandsubinlet_2d[_d]  srcfv/borders/bc_supandsubinlet.F90:2-171,
@@ -140,17 +149,38 @@ int bca_scatter_jv(bca_ctx *ctx, double *jac, int *ia, int *ja, const double *re
 /* set_bndbl_2d(w,field,wbd,im,jm,gh)  set_bnd.f90:2-24 */
 int bca_set_bndbl(bca_ctx *ctx, const double *w, double *field, double *wbd);
 
+/* testvector_partial(wd,m,l,k,gh,im,jm,istart,iend,jstart,jend)
+ * misc/ComputeJacobian.f90:1075-1092: the seeds of (m, l, k) restricted to the
+ * 0-based index box [istart, iend] x [jstart, jend]. */
+int bca_seed_partial(bca_ctx *ctx, double *wd, int m, int l, int k, int istart, int iend, int jstart,
+                     int jend);
+/* computejacobianfromjv_relaxed_withjnandcheck(jac,ia,ja,resd,m,l,k,gh,im,jm,
+ * nbentry,coefdiag,mini,n)  misc/ComputeJacobian.f90:1095-1204: the two-zone
+ * periodic scatter of zone n (0 or 1) into slot block n of a COO of
+ * nbentry = 2*im*jm*(2gh+1)^2*25 entries; slots holding |jac| >= mini are kept
+ * or reset to the dummy row 5*im*jm-2 as the reference does. */
+int bca_scatter_jv_2zones(bca_ctx *ctx, double *jac, int *ia, int *ja, const double *resd, int m, int l,
+                          int k, const double *coefdiag, double mini, int n, long nbentry);
+
 /* ---- batched colored Jacobian --------------------------------------------- */
-/* The whole seed loop of BROADCAST_npz.py:1068-1127 (cylinder.py:1107-1180
- * for the O-mesh): for every seed (m, l, k) of testvector, the linearised BC
+/* The whole seed loop of BROADCAST_npz.py:1068-1127 (cylinder.py:944-985 for
+ * the O-mesh): for every seed (m, l, k) of testvector, the linearised BC
  * sequence `ops` (applied in order) + the tangent residual + the scatter of
  * computejacobianfromjv[_relaxed][_withjn], into the same COO arrays the
- * reference loop fills.  The 5 variables m of one spatial color are carried
- * as 5 tangent directions of one pass.  write_index = 0 skips ia/ja (they
- * depend on the mesh only). */
+ * reference loop fills (a fresh, zeroed COO).  No dense tangent is formed:
+ * face-local Jacobian blocks are computed once and summed per row and seed.
+ * write_index = 0 skips ia/ja (they depend on the mesh only). */
 int bca_jacobian_colored(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
                          const bca_bc_op *ops, int nops, const double *coefdiag, int withjn,
                          double *jac, int *ia, int *ja, long nbentry, int write_index);
+/* The two-zone loop of cylinder.py:895-935 (O-mesh with im not a multiple of
+ * 2gh+1): zones i in [0, isplit-1] and [isplit, im-1] (0-based; the driver
+ * takes isplit = im/2/(2gh+1)*(2gh+1)), seeds of testvector_partial and the
+ * scatter of computejacobianfromjv_relaxed_withjnandcheck on a zeroed COO of
+ * nbentry >= 2*im*jm*(2gh+1)^2*25 entries. */
+int bca_jacobian_colored_2zones(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
+                                const bca_bc_op *ops, int nops, const double *coefdiag, int isplit,
+                                double *jac, int *ia, int *ja, long nbentry, int write_index);
 
 #ifdef __cplusplus
 }

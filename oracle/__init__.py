@@ -111,6 +111,12 @@ class Oracle:
                                        _p(vol), _p(volf), C.c_int(gh), _p(ph),
                                        C.c_int(im), C.c_int(jm))
 
+    def flux_num_dnc5_2d_b(self, res, resb, w, wb, x0, y0, nx, ny, xc, yc, vol, volf, gh,
+                           cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+        ph = self._phys(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4)
+        self.lib.or_flux_num_dnc5_2d_b(_p(resb), _p(wb), _p(w), _p(nx), _p(ny), _p(vol), _p(volf),
+                                       C.c_int(gh), _p(ph), C.c_int(im), C.c_int(jm))
+
     # f_bnd (primal) and f_lin (tangent, wd not None) --------------------------
     def bc_supandsubinlet_2d(self, w, loc, interf, field, nx, ny, gam, im, jm, gh=3, wd=None):
         lm = field.shape[0]
@@ -162,6 +168,17 @@ class Oracle:
         self.lib.or_computejacobianfromjv(_p(jac), _p(ia), _p(ja), _p(resd), C.c_int(m),
                                           C.c_int(l), C.c_int(k), C.c_int(gh), C.c_int(im),
                                           C.c_int(jm), _p(cd), C.c_int(int(withjn)))
+
+    def testvector_partial(self, wd, m, l, k, gh, im, jm, istart, iend, jstart, jend):
+        self.lib.or_testvector_partial(_p(wd), *[C.c_int(v) for v in (m, l, k, gh, im, jm, istart, iend,
+                                                                      jstart, jend)])
+
+    def computejacobianfromjv_relaxed_withjnandcheck(self, jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                                     coefdiag, mini, n):
+        cd = None if coefdiag is None else np.asfortranarray(coefdiag, dtype=np.float64)
+        self.lib.or_computejacobianfromjv_check(_p(jac), _p(ia), _p(ja), _p(resd), C.c_int(m), C.c_int(l),
+                                                C.c_int(k), C.c_int(gh), C.c_int(im), C.c_int(jm), _p(cd),
+                                                C.c_double(mini), C.c_int(n))
 
     def set_bndbl_2d(self, w, field, wbd, im, jm, gh):
         self.lib.or_set_bndbl_2d(_p(w), _p(field), _p(wbd), C.c_int(im), C.c_int(jm), C.c_int(gh))
@@ -286,6 +303,16 @@ class Reference:
             fn = (self.lib.computejacobianfromjv_relaxed_withjn_ if withjn
                   else self.lib.computejacobianfromjv_relaxed_)
             fn(_p(jac), _p(ia), _p(ja), _p(resd), *self._i(m, l, k, gh, im, jm, nb), _p(cd))
+
+    def testvector_partial(self, wd, m, l, k, gh, im, jm, istart, iend, jstart, jend):
+        self.lib.testvector_partial_(_p(wd), *self._i(m, l, k, gh, im, jm, istart, iend, jstart, jend))
+
+    def computejacobianfromjv_relaxed_withjnandcheck(self, jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                                     coefdiag, mini, n):
+        cd = np.asfortranarray(coefdiag, dtype=np.float64)
+        self.lib.computejacobianfromjv_relaxed_withjnandcheck_(
+            _p(jac), _p(ia), _p(ja), _p(resd), *self._i(m, l, k, gh, im, jm, jac.shape[0]), _p(cd),
+            *self._r(mini), *self._i(n))
 
     def set_bndbl_2d(self, w, field, wbd, im, jm, gh):
         self.lib.set_bndbl_2d_(_p(w), _p(field), _p(wbd), *self._i(im, jm, gh))

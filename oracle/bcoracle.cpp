@@ -947,6 +947,51 @@ void or_flux_num_dnc5_2d_d(double* residu, double* residud, const double* w, con
   from_dual(rdual, residu, residud, n);
 }
 
+// flux_num_dnc5_2d_b (srcfv/adjoint/flux_num_dnc5_b.f90:15-6676), restated by
+// transposing colored tangents instead of TAPENADE's reverse sweep.  Every
+// cell of the array (ghost frame included) is an independent input.  The
+// residual of an interior row r depends on cells c with c - r in a window 7
+// cells wide in each direction: i in [-3, 3]; j in [-3, 3] except for the
+// row j = 1, where it is [-2, 4] (the off-centred Euler flux of the face
+// j = 2 reads the cell j = 5).  So under a seed on every cell of one residue
+// class (mod 7 in i and j) of variable m, the tangent of r is dR(r)/dw(c, m)
+// for the single seeded c in its window, and wb(c, m) is the sum over the
+// rows r whose window holds c of resd(r) . resb(r).  wb is overwritten and
+// the interior of residub is zeroed (:809-829, :834).
+void or_flux_num_dnc5_2d_b(double* residub, double* wb, const double* w, const double* nx,
+                           const double* ny, const double* vol, const double* volf, int gh,
+                           const double* phys, int im, int jm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, step = 2 * gh + 1;
+  const long nc = (long)ni * nj, n = nc * 5;
+  std::vector<double> wd(n), res(n), resd(n);
+  for (long q = 0; q < n; ++q) wb[q] = 0.0;
+  auto at = [&](int i, int j) { return (long)(i + gh - 1) + (long)ni * (j + gh - 1); };
+  for (int m = 0; m < 5; ++m)
+    for (int l = 0; l < step; ++l)
+      for (int k = 0; k < step; ++k) {
+        std::fill(wd.begin(), wd.end(), 0.0);
+        std::fill(resd.begin(), resd.end(), 0.0);
+        for (int b = k; b < nj; b += step)
+          for (int a = l; a < ni; a += step) wd[a + (long)ni * b + m * nc] = 1.0;
+        or_flux_num_dnc5_2d_d(res.data(), resd.data(), w, wd.data(), nx, ny, vol, volf, gh, phys, im, jm);
+        for (int b = k; b < nj; b += step)
+          for (int a = l; a < ni; a += step) {
+            const int ci = a - gh + 1, cj = b - gh + 1;
+            double s = 0.0;
+            for (int rj = std::max(1, cj - 4); rj <= std::min(jm, cj + 3); ++rj) {
+              const int dj = cj - rj;
+              if (rj == 1 ? (dj < -2 || dj > 4) : (dj < -3 || dj > 3)) continue;
+              for (int ri = std::max(1, ci - 3); ri <= std::min(im, ci + 3); ++ri)
+                for (int e = 0; e < 5; ++e) s += resd[at(ri, rj) + e * nc] * residub[at(ri, rj) + e * nc];
+            }
+            wb[a + (long)ni * b + m * nc] = s;
+          }
+      }
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i)
+      for (int e = 0; e < 5; ++e) residub[at(i, j) + e * nc] = 0.0;
+}
+
 void or_bc_inlet(double* w, double* wd, int loc, const int* box, const double* field,
                  const double* nx, const double* ny, double gam, int im, int jm, int lm, int gh) {
   Box b{box[0], box[1], box[2], box[3]};
@@ -1056,6 +1101,66 @@ void or_computejacobianfromjv(double* jac, int* ia, int* ja, const double* resd,
         double v = -rd(i, j, e);
         if (coefdiag && ia[cur] == ja[cur]) v = coefdiag[(i - 1) + (long)im * (j - 1)] - rd(i, j, e);
         jac[cur] = v;
+      }
+}
+
+// testvector_partial (misc/ComputeJacobian.f90:1075-1092): the seeds of
+// (m, l, k) restricted to the 0-based index box [istart, iend] x [jstart, jend].
+void or_testvector_partial(double* wd, int m, int l, int k, int gh, int im, int jm, int istart,
+                           int iend, int jstart, int jend) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  std::memset(wd, 0, sizeof(double) * (size_t)ni * nj * 5);
+  F3<double> W{wd, lo, ni, lo, nj, 1};
+  const int step = 1 + 2 * gh;
+  for (int j = jstart + k + 1; j <= jend + 1; j += step)
+    for (int i = istart + l + 1; i <= iend + 1; i += step) W(i, j, m + 1) = 1.0;
+}
+
+// computejacobianfromjv_relaxed_withjnandcheck (misc/ComputeJacobian.f90:
+// 1095-1204): the two-zone periodic scatter of zone n (0: the lower i zone,
+// columns past im-2gh wrap to l; 1: the upper zone, columns below 2gh+1 wrap
+// to im - mod(im, 2gh+1) + l), slot block n of the doubled COO, written only
+// where the slot still holds |jac| < mini.
+void or_computejacobianfromjv_check(double* jac, int* ia, int* ja, const double* resd, int m, int l,
+                                    int k, int gh, int im, int jm, const double* coefdiag, double mini,
+                                    int n) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> rd{resd, lo, ni, lo, nj, 1};
+  const int step = 1 + 2 * gh;
+  const int dummy = 5 * im * jm - 2;
+  for (int e = 1; e <= 5; ++e)
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i) {
+        const long cur = (long)(i - 1) + (long)(j - 1) * im + (long)(e - 1) * im * jm +
+                         (long)k * im * jm * 5 + (long)l * im * jm * 5 * step +
+                         (long)m * im * jm * 5 * step * step + (long)n * im * jm * 5 * 5 * step * step;
+        ia[cur] = e - 1 + (j - 1) * 5 + (i - 1) * jm * 5;
+        const int valj = (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
+        if (valj >= jm) { ia[cur] = dummy; ja[cur] = 0; jac[cur] = 0.0; continue; }
+        int vali;
+        if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
+        else vali = (i <= l - gh) ? l : (i - (l + 1) + gh) / step * step + l;
+        bool wrapped;
+        if (n == 0) {
+          wrapped = vali >= im - 2 * gh;
+          if (wrapped) vali = l;
+        } else {
+          wrapped = vali <= 2 * gh;
+          if (wrapped) {
+            vali = im - im % step + l;
+            if (vali > im - 1) vali -= step;
+          }
+        }
+        const bool empty = std::fabs(jac[cur]) < mini;
+        if (empty) {
+          ja[cur] = m + valj * 5 + vali * jm * 5;
+          jac[cur] = (coefdiag && ia[cur] == ja[cur]) ? coefdiag[(i - 1) + (long)im * (j - 1)] - rd(i, j, e)
+                                                      : -rd(i, j, e);
+        } else if ((n == 0) == wrapped) {
+          // an occupied slot is reset to the dummy on the wrapped columns of
+          // zone 0 and the unwrapped ones of zone 1, and kept otherwise
+          ia[cur] = dummy; ja[cur] = 0; jac[cur] = 0.0;
+        }
       }
 }
 

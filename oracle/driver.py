@@ -98,6 +98,28 @@ def residual(lib, case: Case, g, w):
     return res
 
 
+def jvp_flux(lib, case: Case, g, w, wd):
+    """dR/dw . wd of the flux routine alone: wd's ghost frame as given."""
+    res = np.zeros_like(w, order="F")
+    resd = np.zeros_like(w, order="F")
+    lib.flux_num_dnc5_2d_d(res, resd, np.asfortranarray(w), np.asfortranarray(wd), g["x0"], g["y0"],
+                           g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], case.gh,
+                           *case.phys.scheme_args(), case.im, case.jm)
+    return resd
+
+
+def vjp(lib, case: Case, g, w, resb):
+    """wb = (dR/dw)^T resb of the flux routine alone (cylinder.py:1141);
+    resb is not modified (a copy is consumed)."""
+    rb = np.array(resb, order="F", copy=True)
+    wb = np.zeros_like(rb, order="F")
+    res = np.zeros_like(rb, order="F")
+    lib.flux_num_dnc5_2d_b(res, rb, np.asfortranarray(w), wb, g["x0"], g["y0"], g["nx"], g["ny"],
+                           g["xc"], g["yc"], g["vol"], g["volf"], case.gh, *case.phys.scheme_args(),
+                           case.im, case.jm)
+    return wb
+
+
 def jvp(lib, case: Case, g, w, wd):
     """dR/dw . wd for interior wd: linearised BCs fill wd's ghosts, then the
     tangent residual (BROADCAST_npz.py:1079-1085)."""
@@ -112,28 +134,56 @@ def jvp(lib, case: Case, g, w, wd):
     return res, resd
 
 
-def colored_jacobian(lib, case: Case, g, w, coefdiag=None, prune=2e-16):
-    """The reference's 245-seed assembly (BROADCAST_npz.py:1068-1137): one-hot
-    seeds with period 2gh+1, tangent BCs + JVP per seed, scatter into COO, drop
-    |J| <= prune, then COO -> CSR summing duplicates (misc/PETSc_func.py:71-95).
-    Returns a scipy.sparse.csr_matrix of -dR/dw (+ coefdiag on the diagonal)."""
+def two_zone_split(case: Case):
+    """First i index (0-based) of the upper zone when the reference colors an
+    i-periodic mesh in two zones, None otherwise: an O-mesh whose im is not a
+    multiple of 2gh+1 is split at (im/2)/(2gh+1)*(2gh+1) (cylinder.py:896-897)."""
+    step = 2 * case.gh + 1
+    if case.kind == "cyl" and case.im % step != 0:
+        return (case.im // 2) // step * step
+    return None
+
+
+def colored_jacobian(lib, case: Case, g, w, coefdiag=None, prune=2e-16, coo=False):
+    """The reference's colored assembly (BROADCAST_npz.py:1068-1137;
+    cylinder.py:895-985 for the O-mesh): one-hot seeds with period 2gh+1,
+    tangent BCs + JVP per seed, scatter into COO, drop |J| <= prune, then
+    COO -> CSR summing duplicates (misc/PETSc_func.py:71-95).  An O-mesh
+    with im not a multiple of 2gh+1 uses the two-zone seeds and scatter
+    (testvector_partial, computejacobianfromjv_relaxed_withjnandcheck) over a
+    doubled COO.  Returns a scipy.sparse.csr_matrix of -dR/dw (+ coefdiag on
+    the diagonal), or the raw COO (jac, ia, ja) when coo=True."""
     import scipy.sparse as sp
 
     im, jm, gh = case.im, case.jm, case.gh
     step = 2 * gh + 1
-    nb = im * jm * step * step * 25
+    split = two_zone_split(case)
+    nzones = 1 if split is None else 2
+    nb = im * jm * step * step * 25 * nzones
     jac = np.zeros(nb)
     ia = np.zeros(nb, dtype=np.int32)
     ja = np.zeros(nb, dtype=np.int32)
     withjn = case.kind == "cyl"
     wd = np.zeros_like(w, order="F")
-    for m in range(5):
-        for l in range(step):
-            for k in range(step):
-                lib.testvector(wd, m, l, k, gh, im, jm)
-                _, resd = jvp(lib, case, g, w, wd)
-                lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm,
-                                          coefdiag=coefdiag, withjn=withjn)
+    zones = [(0, im - 1)] if split is None else [(0, split - 1), (split, im - 1)]
+    for n, (istart, iend) in enumerate(zones):
+        for m in range(5):
+            for l in range(step):
+                for k in range(step):
+                    if split is None:
+                        lib.testvector(wd, m, l, k, gh, im, jm)
+                    else:
+                        lib.testvector_partial(wd, m, l, k, gh, im, jm, istart, iend, 0, jm - 1)
+                    _, resd = jvp(lib, case, g, w, wd)
+                    if split is None:
+                        lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                                  coefdiag=coefdiag, withjn=withjn)
+                    else:
+                        cd = coefdiag if coefdiag is not None else np.zeros((im, jm), order="F")
+                        lib.computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, im,
+                                                                         jm, cd, 2e-16, n)
+    if coo:
+        return jac, ia, ja
     keep = np.abs(jac) > prune
     n = 5 * im * jm
     return sp.csr_matrix((jac[keep], (ia[keep], ja[keep])), shape=(n, n))

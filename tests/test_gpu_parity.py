@@ -224,31 +224,27 @@ def test_larger_meshes_match_oracle(torch_mod, O, mk):
 
 
 @pytest.mark.parametrize("mk", [lambda: bl_case(29, 18, seed=31),
-                                lambda: cylinder_case(28, 14, seed=32)])
+                                lambda: cylinder_case(28, 14, seed=32),
+                                lambda: cylinder_case(30, 12, seed=33)])
 def test_batched_jacobian_coo_matches_oracle_loop(torch_mod, O, mk):
-    """The whole COO of bca_jacobian_colored against the oracle running the
-    reference's 245-seed loop (testvector + tangent BCs + JVP + scatter):
+    """The whole COO of the batched assembly against the oracle running the
+    reference's seed loop (testvector + tangent BCs + JVP + scatter; for the
+    O-mesh with im = 30, not a multiple of 7, the two-zone loop of
+    testvector_partial + computejacobianfromjv_relaxed_withjnandcheck):
     indices bit-exact, values to 1e-12 of each row's largest entry."""
     from broadcast_amd.flow import to_device
 
     case = mk()
+    print(f"case {case.kind} {case.im}x{case.jm} two-zone split {D.two_zone_split(case)}", flush=True)
     g = D.geometry(O, case)
     w = np.array(case.w, order="F", copy=True)
     D.apply_bcs(O, case, g, w)
-    im, jm, gh = case.im, case.jm, case.gh
-    nb = im * jm * 49 * 25
+    im, jm = case.im, case.jm
     rng = np.random.default_rng(4)
     cd = np.asfortranarray(rng.uniform(0.5, 1.5, (im, jm)))
-    jac_o, ia_o, ja_o = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
-    wd = np.zeros_like(w, order="F")
-    for m in range(5):
-        for l in range(7):
-            for k in range(7):
-                O.testvector(wd, m, l, k, gh, im, jm)
-                _, resd = D.jvp(O, case, g, w, wd)
-                O.computejacobianfromjv(jac_o, ia_o, ja_o, resd, m, l, k, gh, im, jm, coefdiag=cd,
-                                        withjn=case.kind == "cyl")
+    jac_o, ia_o, ja_o = D.colored_jacobian(O, case, g, w, coefdiag=cd, coo=True)
     f = _flow(case)
+    assert f.nbentry == jac_o.shape[0]
     wg = to_device(case.w)
     f.apply_bcs(wg)
     jac, ia, ja = f.jacobian_colored(wg, coefdiag=to_device(cd))
@@ -261,3 +257,31 @@ def test_batched_jacobian_coo_matches_oracle_loop(torch_mod, O, mk):
     err = np.abs(jac - jac_o)
     bad = err > 1e-12 * np.maximum(rowmax[ia_o], 1e-300)
     assert not bad.any(), (int(bad.sum()), float((err / np.maximum(rowmax[ia_o], 1e-300)).max()))
+
+
+def test_two_zone_seed_and_scatter_match_oracle(torch_mod, O):
+    """testvector_partial + computejacobianfromjv_relaxed_withjnandcheck
+    through f_misc, bit-exact, on slots prefilled with a mix of zeros and
+    values (the scatter keeps or resets occupied slots)."""
+    from broadcast_amd.misc import f_misc
+
+    im, jm, gh = 17, 9, 3
+    ni, nj = im + 2 * gh, jm + 2 * gh
+    rng = np.random.default_rng(6)
+    resd = np.asfortranarray(rng.standard_normal((ni, nj, 5)))
+    cd = np.asfortranarray(rng.standard_normal((im, jm)))
+    for zone in ((0, 6, 0, jm - 1), (7, im - 1, 0, jm - 1), (3, 12, 2, 6)):
+        a, b = np.zeros((ni, nj, 5), order="F"), np.zeros((ni, nj, 5), order="F")
+        f_misc.testvector_partial(a, 2, 4, 1, gh, im, jm, *zone)
+        O.testvector_partial(b, 2, 4, 1, gh, im, jm, *zone)
+        np.testing.assert_array_equal(a, b)
+    nb = im * jm * 49 * 25 * 2
+    pre = np.where(rng.random(nb) < 0.3, rng.standard_normal(nb), 0.0)
+    g = [pre.copy(), np.full(nb, 7, np.int32), np.full(nb, 9, np.int32)]
+    o = [pre.copy(), np.full(nb, 7, np.int32), np.full(nb, 9, np.int32)]
+    for n in (0, 1):
+        for m, l, k in [(0, 0, 0), (2, 5, 3), (4, 6, 6), (1, 3, 0), (3, 2, 5)]:
+            f_misc.computejacobianfromjv_relaxed_withjnandcheck(*g, resd, m, l, k, gh, cd, 2e-16, n)
+            O.computejacobianfromjv_relaxed_withjnandcheck(*o, resd, m, l, k, gh, im, jm, cd, 2e-16, n)
+    for a, b in zip(g, o):
+        np.testing.assert_array_equal(a, b)
