@@ -46,11 +46,14 @@ def kernel_ms(flow, name):
 
 def cpu_baseline(case, budget_s=20.0):
     """The oracle (the C++ restatement of the reference, g++ -O2, serial) on
-    the host: one residual plus a bounded sample of the 245 seed passes,
-    scaled to a full step.  (The flang build of the reference Fortran in
-    oracle/_ref computes the same numbers but runs its residual ~20x slower
-    than the restatement, so it would flatter the GPU; it is not used here.)"""
+    the host: one residual plus a bounded sample of the seed passes of the
+    same coloring (two i-zones for an O-mesh whose im is not a multiple of
+    7), and the prune + scipy CSR of the entries they produced, scaled to a
+    full step.  (The flang build of the reference Fortran in oracle/_ref
+    computes the same numbers but runs its residual ~20x slower than the
+    restatement, so it would flatter the GPU; it is not used here.)"""
     import numpy as np
+    import scipy.sparse as sp
     import oracle
     from oracle import driver as D
 
@@ -63,29 +66,52 @@ def cpu_baseline(case, budget_s=20.0):
     D.residual(lib, case, g, w)
     t_res = time.perf_counter() - t0
     im, jm, gh = case.im, case.jm, case.gh
-    nb = im * jm * 49 * 25
+    split = D.two_zone_split(case)
+    zones = [(0, im - 1)] if split is None else [(0, split - 1), (split, im - 1)]
+    nb = im * jm * 49 * 25 * len(zones)
     jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    cd = np.full((im, jm), 1e-2, order="F")
     wd = np.zeros_like(w, order="F")
-    seeds = [(m, l, k) for m in range(5) for l in range(7) for k in range(7)]
+    seeds = [(n, m, l, k) for n in range(len(zones)) for m in range(5) for l in range(7) for k in range(7)]
     done, t_seed = 0, 0.0
-    for (m, l, k) in seeds:
+    for (n, m, l, k) in seeds:
         t1 = time.perf_counter()
-        lib.testvector(wd, m, l, k, gh, im, jm)
+        if split is None:
+            lib.testvector(wd, m, l, k, gh, im, jm)
+        else:
+            lib.testvector_partial(wd, m, l, k, gh, im, jm, zones[n][0], zones[n][1], 0, jm - 1)
         _, resd = D.jvp(lib, case, g, w, wd)
-        lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm, withjn=case.kind == "cyl")
+        if split is None:
+            lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm, coefdiag=cd,
+                                      withjn=case.kind == "cyl")
+        else:
+            lib.computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, im, jm, cd, 2e-16, n)
         t_seed += time.perf_counter() - t1
         done += 1
         if t_res + t_seed > budget_s:
             break
-    step_s = t_res + t_seed / done * len(seeds)
+    # prune + CSR of the slots the sampled seeds filled (remove_zero_jac,
+    # csr_matrix), scaled by the fraction of slots
+    filled = np.zeros(nb, bool)
+    per = im * jm * 5
+    for (n, m, l, k) in seeds[:done]:
+        s0 = (k * im * jm * 5 + l * im * jm * 5 * 7 + m * im * jm * 5 * 49 + n * im * jm * 5 * 245)
+        filled[s0:s0 + per] = True
+    t2 = time.perf_counter()
+    sub = np.flatnonzero(filled)
+    keep = np.abs(jac[sub]) > 2e-16
+    sp.csr_matrix((jac[sub][keep], (ia[sub][keep], ja[sub][keep])), shape=(per, per))
+    t_csr = (time.perf_counter() - t2) * len(seeds) / done
+    step_s = t_res + t_seed / done * len(seeds) + t_csr
     return {
         "value": im * jm / step_s / 1e6,
         "unit": "Mcells/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"1 BC+residual and {done} of 245 seed passes (testvector + tangent BCs + JVP + "
-                  f"scatter) on the {im}x{jm} O-mesh, serial, scaled to a full step "
-                  f"({t_res:.2f} s residual, {t_seed / done:.2f} s per seed)",
+        "sample": f"1 BC+residual and {done} of {len(seeds)} seed passes (testvector + tangent BCs + JVP + "
+                  f"scatter) and the prune + scipy CSR of their entries, on the {im}x{jm} O-mesh, serial, "
+                  f"scaled to a full step ({t_res:.2f} s residual, {t_seed / done:.2f} s per seed, "
+                  f"{t_csr:.2f} s CSR)",
         "host_cpu": _cpu_model(),
         "nproc": os.cpu_count(),
     }
@@ -139,10 +165,18 @@ def main():
     res = f.new_state()
     bufs = f.jacobian_buffers()
 
+    # relaxation term on the diagonal, as the Newton driver passes it
+    # (BROADCAST_npz.py:1063-1066; its values do not change the work)
+    cd = torch.full((case.im * case.jm,), 1e-2, dtype=torch.float64, device="cuda")
+
+    nzones = 1 if f.two_zone_split is None else 2
+    nseeds = 245 * nzones
+
     def step(first=False):
         f.apply_bcs(w)
         f.residual(w, res)
-        f.jacobian_colored(w, bufs=bufs, write_index=first)
+        f.jacobian_colored(w, coefdiag=cd, bufs=bufs, write_index=first)
+        return f.csr(bufs[0])
 
     print(f"bench: {case.name} cells={cells} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
           flush=True)
@@ -177,7 +211,8 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
     kernels = {}
-    for name in ("k_iface_blocks", "k_jface_blocks", "ghost_tangents", "k_assemble_colored", "k_assemble_ghosts"):
+    for name in ("k_iface_blocks", "k_jface_blocks", "ghost_tangents", "k_assemble_colored", "k_assemble_ghosts",
+                 "k_csr_count", "k_csr_fill"):
         tot, n = kernel_ms(f, name)
         kernels[name] = {"total_ms": tot, "launches": n, "avg_ms": tot / max(n, 1)}
     _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 0), "bca_profile_enable")
@@ -217,9 +252,11 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"cylinder O-mesh {case.im}x{case.jm} (M=0.1, Re_D=50, seeded free stream): "
-                        "BCs + residual + full 245-seed colored Jacobian into COO values",
+                        f"BCs + residual + full colored Jacobian ({nseeds} seeds"
+                        f"{', two i-zones' if nzones == 2 else ''}) into the reference COO, "
+                        "then pruned CSR",
             "cells": cells,
-            "seeds": 245,
+            "seeds": nseeds,
             "jacobian_entries": f.nbentry,
         },
         "roofline": {
@@ -247,7 +284,7 @@ def main():
             "frac": ach_res / HBM_PEAK_GBS,
             "face_kernel_avg_ms": avg64,
         },
-        "jvp_seed_mcells_per_s": 245 * cells / (ms_step * 1e-3) / 1e6,
+        "jvp_seed_mcells_per_s": nseeds * cells / (ms_step * 1e-3) / 1e6,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)

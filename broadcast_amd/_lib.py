@@ -59,6 +59,8 @@ _SIGS = {
     "bca_set_bndbl": [_dp, _dp, _dp, _dp],
     "bca_jacobian_colored": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
                              C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
+    "bca_csr_colored_count": [_dp, _dp, C.c_long, C.c_int, C.c_int, C.c_double, _dp, C.POINTER(C.c_long)],
+    "bca_csr_colored_fill": [_dp, _dp, C.c_long, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp],
     "bca_jacobian_colored_2zones": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
                                     C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
 }

@@ -15,6 +15,7 @@
 #include "misc.h"
 #include "scheme.h"
 #include "vjp.h"
+#include "csr.h"
 
 using namespace bca;
 
@@ -74,6 +75,8 @@ struct bca_ctx {
   Dual<5>* wcol = nullptr;  // seeded states of the zones x 49 colors (frames only)
   unsigned char* ghostnz = nullptr;  // nonzero flags of the ghost tangents
   int zone_cap = 0;         // number of zones the three arrays above hold
+  int* csr_cnt = nullptr;   // row lengths of the pruned CSR
+  int* csr_aux = nullptr;   // scan block sums, total, error flag
   double* vjpi = nullptr;   // VJP contributions per face and offset, i-faces
   double* vjpj = nullptr;   // j-faces
 };
@@ -298,6 +301,8 @@ int bca_ctx_destroy(bca_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->wcol) (void)hipFree(c->wcol);
   if (c->ghostnz) (void)hipFree(c->ghostnz);
+  for (int* p : {c->csr_cnt, c->csr_aux})
+    if (p) (void)hipFree(p);
   delete c;
   return 0;
 }
@@ -679,6 +684,91 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   }
   BCA_LAUNCHED();
   k_zero_interior<<<grid2(m.im, m.jm, b), b, 0, s>>>(m, residub);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- pruned CSR of the colored COO ------------------------------------------
+namespace {
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+int csr_zones(bca_ctx* c, int nzones, int withjn, long nbentry, Zones& z) {
+  if (nzones != 1 && nzones != 2) return fail(-1, "nzones must be 1 or 2");
+  const int step = 2 * c->m.gh + 1;
+  if (nbentry < (long)nzones * c->m.im * c->m.jm * step * step * 25)
+    return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
+  z = Zones{nzones, {1, 0}, {c->m.im, 0}, withjn != 0 || nzones == 2};
+  const long rows = 5L * c->m.im * c->m.jm;
+  if (rows * 245L * nzones > 2147483647L) return fail(-1, "matrix too large for 32-bit CSR indices");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bca_csr_colored_count(bca_ctx* c, const double* jac, long nbentry, int nzones, int withjn, double mini,
+                          int* indptr, long* nnz) {
+  if (!c || !jac || !indptr || !nnz) return fail(-1, "null argument");
+  Zones z;
+  int rc = csr_zones(c, nzones, withjn, nbentry, z);
+  if (rc) return rc;
+  const Mesh& m = c->m;
+  const long rows = 5L * m.im * m.jm;
+  const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
+  if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if (nb > SCAN_B * SCAN_B) return fail(-1, "too many rows for the scan");
+  int* sums = c->csr_aux;
+  int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
+  int* bad = total + 1;
+  hipStream_t s = c->stream;
+  k_fill_int<<<1, 64, 0, s>>>(bad, 1, 0);
+  BCA_LAUNCHED();
+  const dim3 b(64, 4);
+  dim3 gr = grid2(m.im, m.jm, b);
+  gr.z = 5;
+  {
+    Timed t(c, "k_csr_count");
+    k_csr_rows<false><<<gr, b, 0, s>>>(m, z, jac, mini, c->csr_cnt, nullptr, nullptr, nullptr, bad);
+  }
+  BCA_LAUNCHED();
+  k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
+  BCA_LAUNCHED();
+  k_scan_sums<<<1, SCAN_B, 0, s>>>(sums, nb, total);
+  BCA_LAUNCHED();
+  k_scan_add<<<nb, SCAN_B, 0, s>>>(indptr, rows, sums, total);
+  BCA_LAUNCHED();
+  int host[2] = {0, 0};
+  BCA_HIP(hipMemcpyAsync(host, total, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  BCA_HIP(hipStreamSynchronize(s));
+  if (host[1]) return fail(-4, "a kept COO entry has a column outside the matrix");
+  *nnz = host[0];
+  return 0;
+}
+
+int bca_csr_colored_fill(bca_ctx* c, const double* jac, long nbentry, int nzones, int withjn, double mini,
+                         const int* indptr, int* indices, double* data) {
+  if (!c || !jac || !indptr || !indices || !data) return fail(-1, "null argument");
+  Zones z;
+  int rc = csr_zones(c, nzones, withjn, nbentry, z);
+  if (rc) return rc;
+  if (!c->csr_aux) return fail(-1, "bca_csr_colored_count must run first");
+  const Mesh& m = c->m;
+  int* bad = c->csr_aux + (long)SCAN_B * SCAN_B + 1;
+  const dim3 b(64, 4);
+  dim3 gr = grid2(m.im, m.jm, b);
+  gr.z = 5;
+  {
+    Timed t(c, "k_csr_fill");
+    k_csr_rows<true><<<gr, b, 0, c->stream>>>(m, z, jac, mini, nullptr, indptr, indices, data, bad);
+  }
   BCA_LAUNCHED();
   return 0;
 }

@@ -194,13 +194,15 @@ __global__ void k_testvector(Mesh m, double* __restrict__ wd, int mv, int l, int
   wd[c] = on ? 1.0 : 0.0;
 }
 
-// Column cell (0-based vali, valj) that seed color (l, k) attributes to row
-// cell (i, j) under the reference's window rules; false for the dummy slot.
-__host__ __device__ inline bool seed_column(int i, int j, int l, int k, int gh, int im, int jm,
-                                            bool withjn, int& vali, int& valj) {
+// Window rules of the reference's scatter (misc/ComputeJacobian.f90:528-552):
+// the column cell (0-based vali, valj) that seed color (l, k) attributes to
+// row cell (i, j).  valj depends on (j, k) only, vali on (i, l) only.
+__host__ __device__ inline int seed_valj(int j, int k, int gh) {
   const int step = 2 * gh + 1;
-  valj = (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
-  if (valj >= jm) return false;
+  return (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
+}
+__host__ __device__ inline bool seed_vali(int i, int l, int gh, int im, bool withjn, int& vali) {
+  const int step = 2 * gh + 1;
   if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
   else if (i <= l - gh) vali = withjn ? im - 1 - 2 * gh + l : im + 1;
   else vali = (i - (l + 1) + gh) / step * step + l;
@@ -210,6 +212,13 @@ __host__ __device__ inline bool seed_column(int i, int j, int l, int k, int gh, 
   }
   return true;
 }
+// false for the dummy slot
+__host__ __device__ inline bool seed_column(int i, int j, int l, int k, int gh, int im, int jm,
+                                            bool withjn, int& vali, int& valj) {
+  valj = seed_valj(j, k, gh);
+  if (valj >= jm) return false;
+  return seed_vali(i, l, gh, im, withjn, vali);
+}
 
 // Two-zone variant (computejacobianfromjv_relaxed_withjnandcheck,
 // misc/ComputeJacobian.f90:1095-1204) for an empty slot: zone 0 wraps the
@@ -217,12 +226,8 @@ __host__ __device__ inline bool seed_column(int i, int j, int l, int k, int gh, 
 // im - mod(im, 2gh+1) + l (one period lower if that passes im-1).  `wrapped`
 // reports which rule applied (the check resets an occupied wrapped slot of
 // zone 0 and an occupied unwrapped slot of zone 1 to the dummy).
-__host__ __device__ inline bool seed_column_zone(int i, int j, int l, int k, int gh, int im, int jm, int n,
-                                                 int& vali, int& valj, bool& wrapped) {
+__host__ __device__ inline void seed_vali_zone(int i, int l, int gh, int im, int n, int& vali, bool& wrapped) {
   const int step = 2 * gh + 1;
-  valj = (j <= k + 1 + gh) ? k : (j - gh - (k + 1) + 2 * gh) / step * step + k;
-  wrapped = false;
-  if (valj >= jm) return false;
   if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
   else vali = (i <= l - gh) ? l : (i - (l + 1) + gh) / step * step + l;
   if (n == 0) {
@@ -235,6 +240,13 @@ __host__ __device__ inline bool seed_column_zone(int i, int j, int l, int k, int
       if (vali > im - 1) vali -= step;
     }
   }
+}
+__host__ __device__ inline bool seed_column_zone(int i, int j, int l, int k, int gh, int im, int jm, int n,
+                                                 int& vali, int& valj, bool& wrapped) {
+  valj = seed_valj(j, k, gh);
+  wrapped = false;
+  if (valj >= jm) return false;
+  seed_vali_zone(i, l, gh, im, n, vali, wrapped);
   return true;
 }
 

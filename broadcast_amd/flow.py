@@ -235,3 +235,24 @@ class Flow:
                                                     _ptr(ja), self.nbentry, int(write_index))
         _lib.check(rc, "bca_jacobian_colored")
         return jac, ia, ja
+
+    def csr(self, jac, mini=2e-16):
+        """Pruned CSR (indptr, indices, data) of a COO from jacobian_colored:
+        remove_zero_jac + csr_matrix (BROADCAST_npz.py:129-135,
+        misc/PETSc_func.py:85) on the device."""
+        t = self.t
+        n = 5 * self.im * self.jm
+        nz = 1 if self.two_zone_split is None else 2
+        withjn = int(self.case.kind == "cyl")
+        indptr = t.empty(n + 1, dtype=t.int32, device="cuda")
+        nnz = C.c_long()
+        h = self.ctx.sync_stream()
+        if jac.numel() < self.nbentry:
+            raise ValueError(f"jac holds {jac.numel()} values, the colored COO of this mesh {self.nbentry}")
+        _lib.check(self.L.bca_csr_colored_count(h, _ptr(jac), jac.numel(), nz, withjn, float(mini), _ptr(indptr),
+                                                C.byref(nnz)), "bca_csr_colored_count")
+        indices = t.empty(max(nnz.value, 1), dtype=t.int32, device="cuda")
+        data = t.empty(max(nnz.value, 1), dtype=t.float64, device="cuda")
+        _lib.check(self.L.bca_csr_colored_fill(h, _ptr(jac), jac.numel(), nz, withjn, float(mini), _ptr(indptr),
+                                               _ptr(indices), _ptr(data)), "bca_csr_colored_fill")
+        return indptr, indices[:nnz.value], data[:nnz.value]

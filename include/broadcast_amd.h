@@ -97,6 +97,19 @@ int bca_residual_dnc5(bca_ctx *ctx, double *residu, const double *w, bca_metrics
 int bca_jvp_dnc5(bca_ctx *ctx, double *residu, double *residud, const double *w,
                  const double *wd, bca_metrics g, const bca_phys *phys);
 
+/* ---- pruned CSR of the colored COO ---------------------------------------- */
+/* remove_zero_jac (BROADCAST_npz.py:129-135) + scipy.sparse.csr_matrix of
+ * createMatPetscCSR (misc/PETSc_func.py:85) on a COO filled by the colored
+ * assembly (nzones = 1, withjn as passed to it; nzones = 2 for the two-zone
+ * loop; jac holds nbentry >= nzones*im*jm*(2gh+1)^2*25 values): entries with |jac| > mini, duplicate (row, column) pairs summed,
+ * columns ascending in each row.  _count fills indptr (5*im*jm + 1 entries)
+ * and returns nnz (it synchronizes the stream); _fill then writes indices and
+ * data (nnz entries each) from that indptr. */
+int bca_csr_colored_count(bca_ctx *ctx, const double *jac, long nbentry, int nzones, int withjn,
+                          double mini, int *indptr, long *nnz);
+int bca_csr_colored_fill(bca_ctx *ctx, const double *jac, long nbentry, int nzones, int withjn,
+                         double mini, const int *indptr, int *indices, double *data);
+
 /* flux_num_dnc5_2d_b(residu,residub,w,wb,...)
  * replaces srcfv/adjoint/flux_num_dnc5_b.f90:15-6676: wb = (dR/dw)^T residub,
  * with w's ghost frame as independent inputs (no BC adjoint), over the whole

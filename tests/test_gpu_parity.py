@@ -161,6 +161,12 @@ def test_colored_jacobian_matches_reference(torch_mod, name):
     diff = abs(J - Jref).max(axis=1).toarray().ravel()
     rowscale = abs(Jref).max(axis=1).toarray().ravel()
     assert np.all(diff <= 1e-12 * rowscale), np.max(diff / rowscale)
+    # the same through the device prune + CSR
+    indptr, indices, data = f.csr(to_device(jac))
+    Jg = sp.csr_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
+    np.testing.assert_array_equal(Jg.indptr, J.indptr)
+    np.testing.assert_array_equal(Jg.indices, J.indices)
+    np.testing.assert_array_equal(Jg.data, J.data)
 
 
 def test_seed_and_scatter_match_oracle(torch_mod, O):
@@ -257,6 +263,22 @@ def test_batched_jacobian_coo_matches_oracle_loop(torch_mod, O, mk):
     err = np.abs(jac - jac_o)
     bad = err > 1e-12 * np.maximum(rowmax[ia_o], 1e-300)
     assert not bad.any(), (int(bad.sum()), float((err / np.maximum(rowmax[ia_o], 1e-300)).max()))
+    # device prune + CSR: bit-exact against scipy on the oracle's COO values
+    # (remove_zero_jac + csr_matrix, BROADCAST_npz.py:129-135,
+    # misc/PETSc_func.py:85), and end to end within the Jacobian tolerance
+    import scipy.sparse as sp
+
+    keep = np.abs(jac_o) > 2e-16
+    Jo = sp.csr_matrix((jac_o[keep], (ia_o[keep], ja_o[keep])), shape=(n, n))
+    indptr, indices, data = f.csr(to_device(jac_o))
+    np.testing.assert_array_equal(indptr.cpu().numpy(), Jo.indptr)
+    np.testing.assert_array_equal(indices.cpu().numpy(), Jo.indices)
+    np.testing.assert_array_equal(data.cpu().numpy(), Jo.data)
+    indptr, indices, data = f.csr(to_device(jac))
+    Jg = sp.csr_matrix((data.cpu().numpy(), indices.cpu().numpy(), indptr.cpu().numpy()), shape=(n, n))
+    assert Jg.has_sorted_indices
+    diff = abs(Jg - Jo).max(axis=1).toarray().ravel()
+    assert np.all(diff <= 1e-12 * np.maximum(rowmax, 1e-300)), float(np.max(diff / np.maximum(rowmax, 1e-300)))
 
 
 def test_two_zone_seed_and_scatter_match_oracle(torch_mod, O):
