@@ -61,6 +61,12 @@ _SIGS = {
                              C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
     "bca_csr_colored_count": [_dp, _dp, C.c_long, C.c_int, C.c_int, C.c_double, _dp, C.POINTER(C.c_long)],
     "bca_csr_colored_fill": [_dp, _dp, C.c_long, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp],
+    "bca_dz_5p": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_scatter_dz": [_dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_long],
+    "bca_dz_colored": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp, _dp, _dp, _dp,
+                       C.c_long, C.c_int],
+    "bca_jbeta_count": [_dp, _dp, _dp, _dp, C.c_long, C.c_double, _dp, C.POINTER(C.c_long)],
+    "bca_jbeta_fill": [_dp, _dp, _dp, _dp, C.c_long, C.c_double, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_long],
     "bca_jacobian_colored_2zones": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
                                     C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
 }

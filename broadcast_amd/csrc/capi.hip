@@ -16,6 +16,7 @@
 #include "scheme.h"
 #include "vjp.h"
 #include "csr.h"
+#include "dz.h"
 
 using namespace bca;
 
@@ -532,6 +533,34 @@ int bca_set_bndbl(bca_ctx* c, const double* w, double* field, double* wbd) {
 
 namespace {
 
+// T(zone, color, ghost) and its nonzero flags: the tangent the linearised BC
+// sequence gives every ghost cell under each seed color of each zone, all of
+// them per launch, on the boundary frame only.
+int ghost_tangents(bca_ctx* c, const double* w, const bca_metrics& g, const bca_bc_op* ops, int nops,
+                   const Zones& z) {
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1, ncol = step * step, nslot = z.n * ncol;
+  const long ng = ghost_count(m);
+  int rc = 0;
+  if (c->zone_cap < z.n) {
+    c->bytes -= (long)c->zone_cap * ncol * (ng * 25 * (long)sizeof(double) + 5 * m.ncell * (long)sizeof(Dual<5>) + ng);
+    if ((rc = realloc_planes(c, &c->ghostT, (long)nslot * ng * 25))) return rc;
+    if ((rc = realloc_planes(c, &c->wcol, (long)nslot * 5 * m.ncell))) return rc;
+    if ((rc = realloc_planes(c, &c->ghostnz, (long)nslot * ng))) return rc;
+    c->zone_cap = z.n;
+  }
+  hipStream_t s = c->stream;
+  Timed t(c, "ghost_tangents");
+  const long cs = 5 * m.ncell;
+  k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), nslot), 256, 0, s>>>(m, z, w, c->wcol, cs);
+  BCA_LAUNCHED();
+  for (int q = 0; q < nops; ++q)
+    if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, nslot, cs))) return rc;
+  k_ghost_tangents<<<dim3(grid1(ng, 256), nslot), 256, 0, s>>>(m, c->wcol, cs, c->ghostT, c->ghostnz);
+  BCA_LAUNCHED();
+  return 0;
+}
+
 int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
                   const double* coefdiag, const Zones& z, double* jac, int* ia, int* ja, long nbentry,
                   int write_index) {
@@ -545,17 +574,9 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   int rc = 0;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
   const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
-  const long ng = ghost_count(m);
-  const int ncol = step * step, nslot = z.n * ncol;
+  const int nslot = z.n * step * step;
   if ((rc = alloc_planes(c, &c->blki, (long)NOFF * 25 * nfi))) return rc;
   if ((rc = alloc_planes(c, &c->blkj, (long)NOFF * 25 * nfj))) return rc;
-  if (c->zone_cap < z.n) {
-    c->bytes -= (long)c->zone_cap * ncol * (ng * 25 * (long)sizeof(double) + 5 * m.ncell * (long)sizeof(Dual<5>) + ng);
-    if ((rc = realloc_planes(c, &c->ghostT, (long)nslot * ng * 25))) return rc;
-    if ((rc = realloc_planes(c, &c->wcol, (long)nslot * 5 * m.ncell))) return rc;
-    if ((rc = realloc_planes(c, &c->ghostnz, (long)nslot * ng))) return rc;
-    c->zone_cap = z.n;
-  }
   const Metric gm = to_metric(g);
   const Phys ph = to_phys(p);
   hipStream_t s = c->stream;
@@ -588,18 +609,7 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   aj.j0 = aj.j1 = 1;
   launch_jface_blocks_wall(grid2(m.im, 1, b), b, s, aj);
   BCA_LAUNCHED();
-  // ghost tangents of every zone and color through the linearised BC
-  // sequence, all of them per launch
-  {
-    Timed t(c, "ghost_tangents");
-    const long cs = 5 * m.ncell;
-    k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), nslot), 256, 0, s>>>(m, z, w, c->wcol, cs);
-    BCA_LAUNCHED();
-    for (int q = 0; q < nops; ++q)
-      if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, nslot, cs))) return rc;
-    k_ghost_tangents<<<dim3(grid1(ng, 256), nslot), 256, 0, s>>>(m, c->wcol, cs, c->ghostT, c->ghostnz);
-  }
-  BCA_LAUNCHED();
+  if ((rc = ghost_tangents(c, w, g, ops, nops, z))) return rc;
   // phase 2: rows
   {
     Timed t(c, "k_assemble_colored");
@@ -768,6 +778,111 @@ int bca_csr_colored_fill(bca_ctx* c, const double* jac, long nbentry, int nzones
   {
     Timed t(c, "k_csr_fill");
     k_csr_rows<true><<<gr, b, 0, c->stream>>>(m, z, jac, mini, nullptr, indptr, indices, data, bad);
+  }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- spanwise operators and J(beta) -----------------------------------------
+extern "C" {
+
+int bca_dz_5p(bca_ctx* c, double* dz, double* dz2, const double* w, const double* wd, bca_metrics g,
+              const bca_phys* p) {
+  if (!c || !w || !wd || !p || (!dz && !dz2)) return fail(-1, "null argument");
+  if (dz && (!g.nx || !g.ny || !g.vol)) return fail(-1, "null metrics");
+  const Mesh& m = c->m;
+  const dim3 b(64, 4);
+  k_dz<<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, to_phys(p), to_metric(g), w, wd, dz, dz2);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_scatter_dz(bca_ctx* c, double* jac, int* ia, int* ja, const double* dz, int mv, int l, int k,
+                   long nbentry) {
+  if (!c || !jac || !ia || !ja || !dz) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (mv < 0 || mv > 4 || l < 0 || l >= step || k < 0 || k >= step) return fail(-1, "seed index out of range");
+  if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
+  k_scatter_dz<<<grid1(5L * m.im * m.jm, 256), 256, 0, c->stream>>>(m, jac, ia, ja, dz, mv, l, k);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_dz_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+                   double* jdz, double* jdz2, int* ia, int* ja, long nbentry, int write_index) {
+  if (!c || !w || !p || (!jdz && !jdz2) || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+  if (!g.nx || !g.ny || !g.vol) return fail(-1, "null metrics");
+  if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
+  const Zones z{1, {1, 0}, {m.im, 0}, 0};
+  int rc = ghost_tangents(c, w, g, ops, nops, z);
+  if (rc) return rc;
+  const dim3 b(64, 4);
+  dim3 gr = grid2(m.im, m.jm, b);
+  gr.z = step * step;
+  {
+    Timed t(c, "k_dz_colored");
+    k_dz_colored<<<gr, b, 0, c->stream>>>(m, to_phys(p), to_metric(g), w, c->ghostT, c->ghostnz, jdz, jdz2, ia, ja,
+                                          write_index);
+  }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+int bca_jbeta_count(bca_ctx* c, const double* jac, const double* jdz, const double* jdz2, long nbentry,
+                    double mini, int* indptr, long* nnz) {
+  if (!c || !jac || !jdz || !jdz2 || !indptr || !nnz) return fail(-1, "null argument");
+  Zones z;
+  int rc = csr_zones(c, 1, 0, nbentry, z);
+  if (rc) return rc;
+  const Mesh& m = c->m;
+  const long rows = 5L * m.im * m.jm;
+  const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
+  if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  int* sums = c->csr_aux;
+  int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
+  hipStream_t s = c->stream;
+  const dim3 b(64, 4);
+  dim3 gr = grid2(m.im, m.jm, b);
+  gr.z = 5;
+  k_jbeta_rows<false><<<gr, b, 0, s>>>(m, jac, jdz, jdz2, nullptr, nullptr, 0, mini, c->csr_cnt, nullptr, nullptr,
+                                       nullptr, 0);
+  BCA_LAUNCHED();
+  k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
+  BCA_LAUNCHED();
+  k_scan_sums<<<1, SCAN_B, 0, s>>>(sums, nb, total);
+  BCA_LAUNCHED();
+  k_scan_add<<<nb, SCAN_B, 0, s>>>(indptr, rows, sums, total);
+  BCA_LAUNCHED();
+  int host = 0;
+  BCA_HIP(hipMemcpyAsync(&host, total, sizeof(int), hipMemcpyDeviceToHost, s));
+  BCA_HIP(hipStreamSynchronize(s));
+  *nnz = host;
+  return 0;
+}
+
+int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const double* jdz2, long nbentry, double mini,
+                   const double* vol, const double* beta, int nbeta, const int* indptr, int* indices, double* vals,
+                   long nnz) {
+  if (!c || !jac || !jdz || !jdz2 || !vol || !beta || !indptr || !indices || !vals) return fail(-1, "null argument");
+  if (nbeta < 1) return fail(-1, "nbeta must be >= 1");
+  Zones z;
+  int rc = csr_zones(c, 1, 0, nbentry, z);
+  if (rc) return rc;
+  const Mesh& m = c->m;
+  const dim3 b(64, 4);
+  dim3 gr = grid2(m.im, m.jm, b);
+  gr.z = 5;
+  {
+    Timed t(c, "k_jbeta_fill");
+    k_jbeta_rows<true><<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, beta, nbeta, mini, nullptr, indptr, indices,
+                                                vals, nnz);
   }
   BCA_LAUNCHED();
   return 0;

@@ -256,3 +256,43 @@ class Flow:
         _lib.check(self.L.bca_csr_colored_fill(h, _ptr(jac), jac.numel(), nz, withjn, float(mini), _ptr(indptr),
                                                _ptr(indices), _ptr(data)), "bca_csr_colored_fill")
         return indptr, indices[:nnz.value], data[:nnz.value]
+
+    # ------------------------------------------------------------ 3D operators
+    def dz_colored(self, w, bufs=None, write_index=True):
+        """Dz and Dzz of the seed loop of BROADCAST_npz.py:1230-1245 in their
+        COO layout (jdz, jdz2, ia, ja), computejacobianfromdz windows; w must
+        carry its ghost frame."""
+        t = self.t
+        s = 2 * self.gh + 1
+        n = self.im * self.jm * s * s * 25
+        if bufs is None:
+            bufs = (t.zeros(n, dtype=t.float64, device="cuda"), t.zeros(n, dtype=t.float64, device="cuda"),
+                    t.zeros(n, dtype=t.int32, device="cuda"), t.zeros(n, dtype=t.int32, device="cuda"))
+        jdz, jdz2, ia, ja = bufs
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_dz_colored(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops,
+                                         _ptr(jdz), _ptr(jdz2), _ptr(ia), _ptr(ja), n, int(write_index)),
+                   "bca_dz_colored")
+        return jdz, jdz2, ia, ja
+
+    def jbeta(self, jac, jdz, jdz2, betas, mini=2e-16):
+        """J(beta) = Jac/vol(row) - beta^2 Dzz + i beta Dz (BROADCAST_npz.py:1206-1209,
+        biglobal.py:82) for every beta: (indptr, indices, values[nbeta, nnz]
+        complex128) from the COOs of jacobian_colored and dz_colored."""
+        if self.case.kind == "cyl":
+            raise ValueError("J(beta) is composed on the single-zone boundary-layer coloring")
+        t = self.t
+        n = 5 * self.im * self.jm
+        beta = t.as_tensor(list(map(float, betas)), dtype=t.float64, device="cuda")
+        indptr = t.empty(n + 1, dtype=t.int32, device="cuda")
+        nnz = C.c_long()
+        h = self.ctx.sync_stream()
+        nb = jac.numel()
+        _lib.check(self.L.bca_jbeta_count(h, _ptr(jac), _ptr(jdz), _ptr(jdz2), nb, float(mini), _ptr(indptr),
+                                          C.byref(nnz)), "bca_jbeta_count")
+        indices = t.empty(max(nnz.value, 1), dtype=t.int32, device="cuda")
+        vals = t.empty((beta.numel(), max(nnz.value, 1), 2), dtype=t.float64, device="cuda")
+        _lib.check(self.L.bca_jbeta_fill(h, _ptr(jac), _ptr(jdz), _ptr(jdz2), nb, float(mini), _ptr(self.vol),
+                                         _ptr(beta), beta.numel(), _ptr(indptr), _ptr(indices), _ptr(vals),
+                                         nnz.value), "bca_jbeta_fill")
+        return indptr, indices[:nnz.value], t.view_as_complex(vals)[:, :nnz.value]

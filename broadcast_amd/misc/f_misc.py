@@ -73,3 +73,13 @@ def computejacobianfromjv_relaxed_withjn(jac, ia, ja, resd, m, l, k, gh, coefdia
 def computejacobianfromjv_withjn(jac, ia, ja, resd, m, l, k, gh, im, jm, nbentry=None):
     """misc/ComputeJacobian.f90 (join-aware, not relaxed), cylinder.py:1159"""
     _scatter(jac, ia, ja, resd, m, l, k, gh, im, jm, None, True)
+
+
+def computejacobianfromdz(jac, ia, ja, dz, m, l, k, gh, im, jm, nbentry=None):
+    """misc/ComputeJacobian.f90:708-779 (call form of BROADCAST_npz.py:1245)"""
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(jac, ia, ja, dz, out=(0, 1, 2))
+    nb = jac.shape[0] if hasattr(jac, "shape") else len(jac)
+    _lib.check(_lib.lib().bca_scatter_dz(ctx.h, st[0].ptr, st[1].ptr, st[2].ptr, st[3].ptr, int(m), int(l), int(k),
+                                         int(nb)), "computejacobianfromdz")
+    dev.finish(st)

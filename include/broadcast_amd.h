@@ -175,6 +175,38 @@ int bca_seed_partial(bca_ctx *ctx, double *wd, int m, int l, int k, int istart, 
 int bca_scatter_jv_2zones(bca_ctx *ctx, double *jac, int *ia, int *ja, const double *resd, int m, int l,
                           int k, const double *coefdiag, double mini, int n, long nbentry);
 
+/* ---- spanwise operators (3D linearisation) and J(beta) -------------------- */
+/* coeffs_5p_dz(dz,w,wd,...) srcfv/dz/coeffs_5p_dz.F90:10-174 and
+ * coeffs_5p_dz2(dz2,w,wd,...) srcfv/dz/coeffs_5p_dz2.F90: the interior of dz
+ * and/or dz2 (either may be NULL) from the state w and tangent wd (ghost
+ * frames as given).  dz2 does not read the metrics. */
+int bca_dz_5p(bca_ctx *ctx, double *dz, double *dz2, const double *w, const double *wd, bca_metrics g,
+              const bca_phys *phys);
+/* computejacobianfromdz(jac,ia,ja,dz,m,l,k,gh,im,jm,nbentry)
+ * misc/ComputeJacobian.f90:708-779: +dz into the COO slots of seed (m, l, k)
+ * with its symmetric window rules (dummy row 5*im*jm-1). */
+int bca_scatter_dz(bca_ctx *ctx, double *jac, int *ia, int *ja, const double *dz, int m, int l, int k,
+                   long nbentry);
+/* The Dz/Dzz seed loop of BROADCAST_npz.py:1230-1245 in one call: for every
+ * seed (m, l, k), testvector + the linearised BC sequence `ops` + coeffs_5p_dz
+ * and coeffs_5p_dz2 + computejacobianfromdz, into the COO values jdz and/or
+ * jdz2 (either may be NULL; ia/ja are the same for both). */
+int bca_dz_colored(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys, const bca_bc_op *ops,
+                   int nops, double *jdz, double *jdz2, int *ia, int *ja, long nbentry, int write_index);
+/* J(beta) = Jac/vol(row) - beta^2 Dzz + i beta Dz (BROADCAST_npz.py:1206-1209,
+ * biglobal.py:82) for nbeta wavenumbers, from three COOs on the slot layout
+ * of the plain/relaxed single-zone colored Jacobian (jac from
+ * bca_jacobian_colored with withjn = 0, jdz and jdz2 from bca_dz_colored):
+ * each pruned at |x| > mini like remove_zero_jac, summed over the union of
+ * their patterns.  _count fills indptr (5*im*jm + 1) and returns nnz (it
+ * synchronizes the stream); _fill writes the shared column indices and
+ * vals[b][nnz] as (re, im) pairs (complex128). */
+int bca_jbeta_count(bca_ctx *ctx, const double *jac, const double *jdz, const double *jdz2, long nbentry,
+                    double mini, int *indptr, long *nnz);
+int bca_jbeta_fill(bca_ctx *ctx, const double *jac, const double *jdz, const double *jdz2, long nbentry,
+                   double mini, const double *vol, const double *beta, int nbeta, const int *indptr,
+                   int *indices, double *vals, long nnz);
+
 /* ---- batched colored Jacobian --------------------------------------------- */
 /* The whole seed loop of BROADCAST_npz.py:1068-1127 (cylinder.py:944-985 for
  * the O-mesh): for every seed (m, l, k) of testvector, the linearised BC

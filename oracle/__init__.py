@@ -169,6 +169,21 @@ class Oracle:
                                           C.c_int(l), C.c_int(k), C.c_int(gh), C.c_int(im),
                                           C.c_int(jm), _p(cd), C.c_int(int(withjn)))
 
+    def coeffs_5p_dz(self, dz, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam, rgaz,
+                     cs, muref, tref, s_suth, im, jm):
+        ph = self._phys(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, 0.0, 0.0)
+        self.lib.or_coeffs_5p_dz(_p(dz), _p(w), _p(wd), _p(nx), _p(ny), _p(vol), _p(volf), C.c_int(gh),
+                                 _p(ph), C.c_int(im), C.c_int(jm))
+
+    def coeffs_5p_dz2(self, dz2, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam, rgaz,
+                      cs, muref, tref, s_suth, im, jm):
+        ph = self._phys(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, 0.0, 0.0)
+        self.lib.or_coeffs_5p_dz2(_p(dz2), _p(w), _p(wd), C.c_int(gh), _p(ph), C.c_int(im), C.c_int(jm))
+
+    def computejacobianfromdz(self, jac, ia, ja, dz, m, l, k, gh, im, jm):
+        self.lib.or_computejacobianfromdz(_p(jac), _p(ia), _p(ja), _p(dz), *[C.c_int(v) for v in
+                                                                             (m, l, k, gh, im, jm)])
+
     def testvector_partial(self, wd, m, l, k, gh, im, jm, istart, iend, jstart, jend):
         self.lib.or_testvector_partial(_p(wd), *[C.c_int(v) for v in (m, l, k, gh, im, jm, istart, iend,
                                                                       jstart, jend)])
@@ -303,6 +318,22 @@ class Reference:
             fn = (self.lib.computejacobianfromjv_relaxed_withjn_ if withjn
                   else self.lib.computejacobianfromjv_relaxed_)
             fn(_p(jac), _p(ia), _p(ja), _p(resd), *self._i(m, l, k, gh, im, jm, nb), _p(cd))
+
+    def coeffs_5p_dz(self, dz, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam, rgaz,
+                     cs, muref, tref, s_suth, im, jm):
+        args = [_p(dz), _p(w), _p(wd), _p(x0), _p(y0), _p(nx), _p(ny), _p(xc), _p(yc), _p(vol), _p(volf),
+                *self._i(gh), *self._r(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth), *self._i(im, jm)]
+        _run_big_stack(lambda: self.lib.coeffs_5p_dz_(*args))
+
+    def coeffs_5p_dz2(self, dz2, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam, rgaz,
+                      cs, muref, tref, s_suth, im, jm):
+        args = [_p(dz2), _p(w), _p(wd), _p(x0), _p(y0), _p(nx), _p(ny), _p(xc), _p(yc), _p(vol), _p(volf),
+                *self._i(gh), *self._r(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth), *self._i(im, jm)]
+        _run_big_stack(lambda: self.lib.coeffs_5p_dz2_(*args))
+
+    def computejacobianfromdz(self, jac, ia, ja, dz, m, l, k, gh, im, jm):
+        self.lib.computejacobianfromdz_(_p(jac), _p(ia), _p(ja), _p(dz), *self._i(m, l, k, gh, im, jm,
+                                                                                  jac.shape[0]))
 
     def testvector_partial(self, wd, m, l, k, gh, im, jm, istart, iend, jstart, jend):
         self.lib.testvector_partial_(_p(wd), *self._i(m, l, k, gh, im, jm, istart, iend, jstart, jend))

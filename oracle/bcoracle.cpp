@@ -1164,6 +1164,252 @@ void or_computejacobianfromjv_check(double* jac, int* ia, int* ja, const double*
       }
 }
 
+// coeffs_5p_dz / coeffs_5p_dz2 (srcfv/dz/coeffs_5p_dz.F90:10-174,
+// coeffs_5p_dz2.F90; expanded srcfv/prepro_dz/coeffs_5p_dz{,2}.f90): the
+// spanwise-derivative operators of the 3D linearisation, dz = sum_n
+// coeffs(e, n) * d func_n(e) / dw . wd (16 terms) and dz2 the same with 4
+// pointwise terms.  Statement order follows the expanded sources: primal
+// primitives (mu from phys/viscosity.F) and 4th-order gradients for the
+// coefficients; the tangent primitives, tangent gradients and the func*d of
+// dz/function_5p_dz{,2}_d.f90; then the sums of dz/sum_coeffs_dz{,2}.F.
+namespace {
+struct DzWork {
+  int ni, nj, lo;
+  std::vector<double> velx, vely, velz, tloc, p, mu;                // coefficients' primal
+  std::vector<double> fvelz, fmu;                                   // the function's primal
+  std::vector<double> velxd, velyd, velzd, tlocd, pd, mud;          // tangents
+  long at(int i, int j) const { return (long)(i - lo) + (long)ni * (j - lo); }
+};
+
+void dz_primitives(DzWork& d, const double* w_p, const double* wd_p, const Phys& ph, int gh, int im, int jm,
+                   bool with_mu_tangent) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  d.ni = ni; d.nj = nj; d.lo = lo;
+  const long nc = (long)ni * nj;
+  for (auto* v : {&d.velx, &d.vely, &d.velz, &d.tloc, &d.p, &d.mu, &d.fvelz, &d.fmu, &d.velxd, &d.velyd,
+                  &d.velzd, &d.tlocd, &d.pd, &d.mud})
+    v->assign(nc, 0.0);
+  F3<const double> w{w_p, lo, ni, lo, nj, 1};
+  F3<const double> wd{wd_p, lo, ni, lo, nj, 1};
+  const double ONE = 1.0, HALF = 0.5;
+  const double cvm1 = ONE / ph.cv;
+  const double betas = ph.muref * (ph.tref + ph.cs) / (std::sqrt(ph.tref) * ph.tref);
+  for (int j = lo; j <= jm + gh; ++j)
+    for (int i = lo; i <= im + gh; ++i) {
+      const long c = d.at(i, j);
+      // rhs/primvisc.F (phys/Primitives.F, phys/viscosity.F)
+      double ro = w(i, j, 1), rom1 = ONE / ro;
+      d.velx[c] = w(i, j, 2) * rom1;
+      d.vely[c] = w(i, j, 3) * rom1;
+      d.velz[c] = w(i, j, 4) * rom1;
+      double ec = HALF * (d.velx[c] * d.velx[c] + d.vely[c] * d.vely[c] + d.velz[c] * d.velz[c]);
+      double eloc = (w(i, j, 5) - ec * ro) * rom1;
+      d.tloc[c] = eloc * cvm1;
+      d.p[c] = (ph.gam - ONE) * ro * eloc;
+      d.mu[c] = betas / (d.tloc[c] + ph.s_suth) * std::sqrt(d.tloc[c]) * d.tloc[c];
+      // the tangent function's own statements
+      const double rod = wd(i, j, 1);
+      const double rom1d = -(ONE * rod / (ro * ro));
+      d.velxd[c] = rom1 * wd(i, j, 2) + w(i, j, 2) * rom1d;
+      d.velyd[c] = rom1 * wd(i, j, 3) + w(i, j, 3) * rom1d;
+      d.velzd[c] = rom1 * wd(i, j, 4) + w(i, j, 4) * rom1d;
+      d.fvelz[c] = w(i, j, 4) * rom1;
+      const double ecd = HALF * (2 * d.velx[c] * d.velxd[c] + 2 * d.vely[c] * d.velyd[c] +
+                                 2 * d.velz[c] * d.velzd[c]);
+      double temp = w(i, j, 5) - ec * ro;
+      const double elocd = rom1 * (wd(i, j, 5) - ro * ecd - ec * rod) + temp * rom1d;
+      eloc = temp * rom1;
+      d.tlocd[c] = cvm1 * elocd;
+      const double tl = eloc * cvm1;
+      d.pd[c] = (ph.gam - ONE) * (eloc * rod + ro * elocd);
+      if (with_mu_tangent) {
+        temp = std::sqrt(tl);
+        const double r1d = tl == 0.0 ? 0.0 : d.tlocd[c] / (2.0 * temp);
+        const double r1 = temp;
+        temp = r1 * tl / (ph.s_suth + tl);
+        d.mud[c] = betas * (tl * r1d + (r1 - temp) * d.tlocd[c]) / (ph.s_suth + tl);
+        d.fmu[c] = betas * temp;
+      }
+    }
+}
+}  // namespace
+
+void or_coeffs_5p_dz(double* dz_p, const double* w_p, const double* wd_p, const double* nx_p, const double* ny_p,
+                     const double* vol_p, const double* volf_p, int gh, const double* phys, int im, int jm) {
+  (void)volf_p;
+  Phys ph{phys[0], phys[1], phys[2], phys[3], phys[4], phys[5], phys[6], phys[7], phys[8], phys[9], phys[10]};
+  DzWork d;
+  dz_primitives(d, w_p, wd_p, ph, gh, im, jm, true);
+  const int ni = d.ni, nj = d.nj, lo = d.lo;
+  F3<const double> w{w_p, lo, ni, lo, nj, 1};
+  F3<const double> wd{wd_p, lo, ni, lo, nj, 1};
+  F3<const double> nx{nx_p, lo, ni + 1, lo, nj + 1, 1};
+  F3<const double> ny{ny_p, lo, ni + 1, lo, nj + 1, 1};
+  F2<const double> vol{vol_p, lo, ni, lo};
+  F3<double> dz{dz_p, lo, ni, lo, nj, 1};
+  const double ONE = 1.0, HALF = 0.5, TWOTHIRD = 2.0 / 3.0, ZERO = 0.0;
+  const double denom = 1.0 / 12.0, b1 = 8.0 * denom, b2 = -denom;
+  auto V = [&](const std::vector<double>& a, int i, int j) { return a[d.at(i, j)]; };
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i) {
+      auto g4 = [&](const std::vector<double>& a, int di, int dj) {
+        return b1 * (V(a, i + di, j + dj) - V(a, i - di, j - dj)) + b2 * (V(a, i + 2 * di, j + 2 * dj) -
+                                                                         V(a, i - 2 * di, j - 2 * dj));
+      };
+      const double volm1 = ONE / vol(i, j);
+      const double dxm1 = HALF * (nx(i, j, 1) + nx(i + 1, j, 1)) * volm1;
+      const double dxm2 = HALF * (nx(i, j, 2) + nx(i, j + 1, 2)) * volm1;
+      const double dym1 = HALF * (ny(i, j, 1) + ny(i + 1, j, 1)) * volm1;
+      const double dym2 = HALF * (ny(i, j, 2) + ny(i, j + 1, 2)) * volm1;
+      // primal gradients of the coefficients (rhs/gradop_5pi.F, gradient.F)
+      const double gui = g4(d.velx, 1, 0), gvi = g4(d.vely, 1, 0), gwi = g4(d.velz, 1, 0), gmui = g4(d.mu, 1, 0);
+      const double guj = g4(d.velx, 0, 1), gvj = g4(d.vely, 0, 1), gwj = g4(d.velz, 0, 1), gmuj = g4(d.mu, 0, 1);
+      const double gradu1 = dxm1 * gui + dxm2 * guj, gradv1 = dxm1 * gvi + dxm2 * gvj;
+      const double gradw1 = dxm1 * gwi + dxm2 * gwj, gradmu1 = dxm1 * gmui + dxm2 * gmuj;
+      const double gradu2 = dym1 * gui + dym2 * guj, gradv2 = dym1 * gvi + dym2 * gvj;
+      const double gradw2 = dym1 * gwi + dym2 * gwj, gradmu2 = dym1 * gmui + dym2 * gmuj;
+      // tangent gradients (function_5p_dz_d.f90:214-262)
+      const double guid = g4(d.velxd, 1, 0), gvid = g4(d.velyd, 1, 0), gwid = g4(d.velzd, 1, 0);
+      const double gujd = g4(d.velxd, 0, 1), gvjd = g4(d.velyd, 0, 1), gwjd = g4(d.velzd, 0, 1);
+      const double gradud1 = dxm1 * guid + dxm2 * gujd, gradvd1 = dxm1 * gvid + dxm2 * gvjd;
+      const double gradwd1 = dxm1 * gwid + dxm2 * gwjd;
+      const double gradud2 = dym1 * guid + dym2 * gujd, gradvd2 = dym1 * gvid + dym2 * gvjd;
+      const double gradwd2 = dym1 * gwid + dym2 * gwjd;
+      const long c = d.at(i, j);
+      const double velx = d.velx[c], vely = d.vely[c], velz = d.velz[c], mu = d.mu[c];
+      const double fvelz = d.fvelz[c], fmu = d.fmu[c], mud = d.mud[c];
+      const double velxd = d.velxd[c], velyd = d.velyd[c], velzd = d.velzd[c], pd = d.pd[c];
+      // func*d (function_5p_dz_d.f90:281-400), by variable e
+      double f[16][5];
+      for (auto& row : f)
+        for (double& x : row) x = 0.0;
+      f[0][0] = wd(i, j, 4);
+      f[0][1] = fvelz * wd(i, j, 2) + w(i, j, 2) * velzd - gradw1 * mud - fmu * gradwd1;
+      f[0][2] = fvelz * wd(i, j, 3) + w(i, j, 3) * velzd - gradw2 * mud - fmu * gradwd2;
+      double temp = gradu1 + gradv2;
+      f[0][3] = fvelz * wd(i, j, 4) + w(i, j, 4) * velzd + pd +
+                TWOTHIRD * (temp * mud + fmu * (gradud1 + gradvd2));
+      temp = w(i, j, 5) + d.p[c];
+      f[0][4] = fvelz * (wd(i, j, 5) + pd) + temp * velzd;
+      f[1][1] = velzd; f[1][2] = velzd; f[1][3] = velxd; f[1][4] = velzd;
+      f[2][1] = gradwd1; f[2][2] = gradwd2; f[2][3] = velyd; f[2][4] = gradwd1;
+      f[3][3] = gradud1 + gradvd2; f[3][4] = velxd;
+      f[4][4] = gradud1;
+      f[5][4] = velzd;
+      f[6][4] = gradwd2;
+      f[7][4] = velyd;
+      f[8][4] = gradvd2;
+      f[9][4] = velxd;
+      f[10][4] = gradw1 * mud + fmu * gradwd1;
+      f[11][4] = velyd;
+      f[12][4] = gradw2 * mud + fmu * gradwd2;
+      f[13][4] = velzd;
+      f[14][4] = mud;
+      f[15][4] = gradud1 + gradvd2;
+      // coefficients (dz/coeffs_dz.F), by variable e and term n
+      double co[5][16];
+      for (auto& row : co)
+        for (double& x : row) x = ZERO;
+      for (int e = 0; e < 5; ++e) co[e][0] = ONE;
+      co[1][1] = TWOTHIRD * gradmu1;
+      co[2][1] = TWOTHIRD * gradmu2;
+      co[3][1] = -gradmu1;
+      co[4][1] = TWOTHIRD * (gradmu1 * velx + mu * gradu1);
+      co[1][2] = TWOTHIRD * mu;
+      co[2][2] = TWOTHIRD * mu;
+      co[3][2] = -gradmu2;
+      co[4][2] = TWOTHIRD * mu * velx;
+      co[3][3] = -mu;
+      co[4][3] = -(gradmu1 * velz + mu * gradw1);
+      co[4][4] = -mu * velz;
+      co[4][5] = TWOTHIRD * (gradmu2 * vely + mu * gradv2);
+      co[4][6] = TWOTHIRD * mu * vely;
+      co[4][7] = -(gradmu2 * velz + mu * gradw2);
+      co[4][8] = -mu * velz;
+      co[4][9] = -mu * gradw1;
+      co[4][10] = -velx;
+      co[4][11] = -mu * gradw2;
+      co[4][12] = -vely;
+      co[4][13] = TWOTHIRD * mu * (gradu1 + gradv2);
+      co[4][14] = TWOTHIRD * velz * (gradu1 + gradv2);
+      co[4][15] = TWOTHIRD * mu * velz;
+      // dz/sum_coeffs_dz.F
+      for (int e = 0; e < 5; ++e) {
+        double s = co[e][0] * f[0][e];
+        for (int n = 1; n < 16; ++n) s = s + co[e][n] * f[n][e];
+        dz(i, j, e + 1) = s;
+      }
+    }
+}
+
+void or_coeffs_5p_dz2(double* dz2_p, const double* w_p, const double* wd_p, int gh, const double* phys, int im,
+                      int jm) {
+  Phys ph{phys[0], phys[1], phys[2], phys[3], phys[4], phys[5], phys[6], phys[7], phys[8], phys[9], phys[10]};
+  DzWork d;
+  dz_primitives(d, w_p, wd_p, ph, gh, im, jm, false);
+  const int ni = d.ni, nj = d.nj, lo = d.lo;
+  F3<double> dz2{dz2_p, lo, ni, lo, nj, 1};
+  const double TWO = 2.0, TWOTHIRD = 2.0 / 3.0, ZERO = 0.0;
+  const double cpprandtl = ph.cp / ph.prandtl;
+  for (int j = 1; j <= jm; ++j)
+    for (int i = 1; i <= im; ++i) {
+      const long c = d.at(i, j);
+      const double mu = d.mu[c];
+      // dz/coeffs_dz2.F
+      double co[5][4];
+      for (auto& row : co)
+        for (double& x : row) x = ZERO;
+      co[1][0] = -mu;
+      co[2][0] = -mu;
+      co[3][0] = -(TWO * TWOTHIRD * mu);
+      co[4][0] = -mu * cpprandtl;
+      co[4][1] = -mu * d.velx[c];
+      co[4][2] = -mu * d.vely[c];
+      co[4][3] = -(TWO * TWOTHIRD * mu * d.velz[c]);
+      // function_5p_dz2_d.f90:160-184
+      double f[4][5];
+      for (auto& row : f)
+        for (double& x : row) x = 0.0;
+      f[0][1] = d.velxd[c]; f[0][2] = d.velyd[c]; f[0][3] = d.velzd[c]; f[0][4] = d.tlocd[c];
+      f[1][4] = d.velxd[c];
+      f[2][4] = d.velyd[c];
+      f[3][4] = d.velzd[c];
+      // dz/sum_coeffs_dz2.F
+      for (int e = 0; e < 5; ++e) {
+        double s = co[e][0] * f[0][e];
+        for (int n = 1; n < 4; ++n) s = s + co[e][n] * f[n][e];
+        dz2(i, j, e + 1) = s;
+      }
+    }
+}
+
+// computejacobianfromdz (misc/ComputeJacobian.f90:708-779): +dz into the
+// COO slots of seed (m, l, k), window rules symmetric in i and j, dummy row
+// 5*im*jm-1.
+void or_computejacobianfromdz(double* jac, int* ia, int* ja, const double* dz, int m, int l, int k, int gh,
+                              int im, int jm) {
+  const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;
+  F3<const double> d{dz, lo, ni, lo, nj, 1};
+  const int step = 1 + 2 * gh;
+  for (int e = 1; e <= 5; ++e)
+    for (int j = 1; j <= jm; ++j)
+      for (int i = 1; i <= im; ++i) {
+        const long cur = (long)(i - 1) + (long)(j - 1) * im + (long)(e - 1) * im * jm + (long)k * im * jm * 5 +
+                         (long)l * im * jm * 5 * step + (long)m * im * jm * 5 * step * step;
+        ia[cur] = e - 1 + (j - 1) * 5 + (i - 1) * jm * 5;
+        int valj;
+        if (k <= gh) valj = (j <= k + 1 + gh) ? k : (j - (k + 1) + gh) / step * step + k;
+        else valj = (j <= k - gh) ? jm + 1 : (j - (k + 1) + gh) / step * step + k;
+        if (valj >= jm) { ia[cur] = 5 * im * jm - 1; ja[cur] = 0; jac[cur] = 0.0; continue; }
+        int vali;
+        if (l <= gh) vali = (i <= l + 1 + gh) ? l : (i - (l + 1) + gh) / step * step + l;
+        else vali = (i <= l - gh) ? im + 1 : (i - (l + 1) + gh) / step * step + l;
+        if (vali >= im) { ia[cur] = 5 * im * jm - 1; ja[cur] = 0; jac[cur] = 0.0; continue; }
+        ja[cur] = m + valj * 5 + vali * jm * 5;
+        jac[cur] = d(i, j, e);
+      }
+}
+
 // set_bndbl_2d (set_bnd.f90:2-24)
 void or_set_bndbl_2d(const double* w, double* field, double* wbd, int im, int jm, int gh) {
   const int ni = im + 2 * gh, nj = jm + 2 * gh, lo = 1 - gh;

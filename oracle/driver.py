@@ -205,3 +205,38 @@ def brute_jacobian(lib, case: Case, g, w):
                 col = (i * jm + j) * 5 + m
                 J[:, col] = -np.ravel(resd[gh:gh + im, gh:gh + jm, :])
     return J
+
+
+def dz_operators(lib, case: Case, g, w, wd):
+    """coeffs_5p_dz and coeffs_5p_dz2 (srcfv/dz) of wd after the linearised
+    BCs (BROADCAST_npz.py:1236-1243)."""
+    w = np.array(w, order="F", copy=True)
+    wd = np.array(wd, order="F", copy=True)
+    apply_bcs(lib, case, g, w, wd=wd)
+    dz = np.zeros_like(w, order="F")
+    dz2 = np.zeros_like(w, order="F")
+    a = case.phys.scheme_args()[:9]
+    for fn, out in ((lib.coeffs_5p_dz, dz), (lib.coeffs_5p_dz2, dz2)):
+        fn(out, w, wd, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], case.gh, *a,
+           case.im, case.jm)
+    return dz, dz2
+
+
+def dz_colored(lib, case: Case, g, w):
+    """The Dz/Dzz seed loop of BROADCAST_npz.py:1230-1245: raw COOs
+    (jdz, jdz2, ia, ja) before remove_zero_jac."""
+    im, jm, gh = case.im, case.jm, case.gh
+    nb = im * jm * 49 * 25
+    jdz, jdz2 = np.zeros(nb), np.zeros(nb)
+    ia, ja = np.zeros(nb, dtype=np.int32), np.zeros(nb, dtype=np.int32)
+    ia2, ja2 = np.zeros(nb, dtype=np.int32), np.zeros(nb, dtype=np.int32)
+    wd = np.zeros_like(w, order="F")
+    for m in range(5):
+        for l in range(7):
+            for k in range(7):
+                lib.testvector(wd, m, l, k, gh, im, jm)
+                dz, dz2 = dz_operators(lib, case, g, w, wd)
+                lib.computejacobianfromdz(jdz, ia, ja, dz, m, l, k, gh, im, jm)
+                lib.computejacobianfromdz(jdz2, ia2, ja2, dz2, m, l, k, gh, im, jm)
+    assert np.array_equal(ia, ia2) and np.array_equal(ja, ja2)
+    return jdz, jdz2, ia, ja

@@ -8,6 +8,9 @@ repository root:
 
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
 
+`--dz-only` adds (or refreshes) the spanwise operators dz, dz2 of the case
+fixtures from their stored inputs without regenerating anything else.
+
 Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
 """
 from __future__ import annotations
@@ -72,6 +75,22 @@ def case_fixture(R, case, seed):
     )
 
 
+def dz_vectors(R, fx):
+    """coeffs_5p_dz / coeffs_5p_dz2 (srcfv/dz) on the stored state after BCs
+    and the first tangent direction after the linearised BCs."""
+    g = {k: np.asfortranarray(fx[f"geo_{k}"]) for k in ("x0", "y0", "nx", "ny", "xc", "yc", "vol", "volf")}
+    w = np.asfortranarray(fx["w_bc"])
+    wd = np.asfortranarray(fx["wd_bc"][0])
+    args = tuple(float(v) for v in fx["phys"][:9])
+    gh, im, jm = int(fx["gh"]), int(fx["im"]), int(fx["jm"])
+    dz = np.zeros_like(w, order="F")
+    dz2 = np.zeros_like(w, order="F")
+    for fn, out in ((R.coeffs_5p_dz, dz), (R.coeffs_5p_dz2, dz2)):
+        fn(out, w, wd, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], gh, *args,
+           im, jm)
+    return dict(dz=dz, dz2=dz2)
+
+
 def jac_fixture(R, case):
     g = D.geometry(R, case)
     w = np.array(case.w, order="F", copy=True)
@@ -88,8 +107,19 @@ def jac_fixture(R, case):
                 indices=J.indices.astype(np.int32), data=J.data)
 
 
+CASES = ("bl24x16", "bl13x9", "cyl28x12")
+
+
 def main():
     R = oracle.Reference()
+    if "--dz-only" in sys.argv:
+        for name in CASES:
+            path = os.path.join(OUT, f"{name}.npz")
+            fx = dict(np.load(path, allow_pickle=False))
+            fx.update(dz_vectors(R, fx))
+            np.savez_compressed(path, **fx)
+            print(f"updated {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+        return
     fixtures = {
         "bl24x16": case_fixture(R, bl_case(24, 16, seed=1), 11),
         "bl13x9": case_fixture(R, bl_case(13, 9, seed=3, reynolds=3e5), 13),
@@ -97,6 +127,8 @@ def main():
         "jac_bl14x10": jac_fixture(R, bl_case(14, 10, seed=4)),
         "jac_cyl14x8": jac_fixture(R, cylinder_case(14, 8, seed=5)),
     }
+    for name in CASES:
+        fixtures[name].update(dz_vectors(R, fixtures[name]))
     for name, fx in fixtures.items():
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **fx)

@@ -276,3 +276,113 @@ def test_two_zone_seeds_and_scatter_match_reference(O):
         outs.append((jac, ia, ja))
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
+
+
+# ------------------------------------------------------------------ Dz, Dzz
+def _uniform(case, U=0.3):
+    """rho = 1, u = U, v = w = 0 at the free-stream pressure everywhere."""
+    w = np.zeros((case.ni, case.nj, 5), order="F")
+    p = case.phys.p_inf
+    w[:, :, 0] = 1.0
+    w[:, :, 1] = U
+    w[:, :, 4] = p / (case.phys.gam - 1.0) + 0.5 * U * U
+    return w
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dz_vs_reference(O, name):
+    fx = load(name)
+    case = case_from(fx)
+    g = {k: np.asfortranarray(fx[f"geo_{k}"]) for k in ("x0", "y0", "nx", "ny", "xc", "yc", "vol", "volf")}
+    a = case.phys.scheme_args()[:9]
+    for fn, key in ((O.coeffs_5p_dz, "dz"), (O.coeffs_5p_dz2, "dz2")):
+        out = np.zeros_like(fx["w_bc"], order="F")
+        fn(out, np.asfortranarray(fx["w_bc"]), np.asfortranarray(fx["wd_bc"][0]), g["x0"], g["y0"], g["nx"],
+           g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], case.gh, *a, case.im, case.jm)
+        ok, err = close(out, fx[key], 1e-13)
+        assert ok, (key, err)
+
+
+def test_dz_closed_forms(O):
+    """Uniform rho = 1, u = U state: Dz of a unit spanwise-momentum tangent is
+    the z-flux Jacobian column (1, u, v, 0, H); Dzz of a unit streamwise
+    momentum tangent is (0, -mu, 0, 0, mu U (gamma/Pr - 1)) (viscous and
+    conductive z-diffusion of u and of T = e/cv)."""
+    case = bl_case(12, 9, seed=0, ratio=1.1, dy1=1e-2)
+    g = D.geometry(O, case)
+    U = 0.3
+    w = _uniform(case, U)
+    a = case.phys.scheme_args()
+    ph = case.phys
+    e4 = np.zeros_like(w, order="F")
+    e4[:, :, 3] = 1.0
+    dz = np.zeros_like(w, order="F")
+    O.coeffs_5p_dz(dz, w, e4, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], 3,
+                   *a[:9], case.im, case.jm)
+    H = w[0, 0, 4] + ph.p_inf
+    np.testing.assert_allclose(dz[3:-3, 3:-3].reshape(-1, 5), np.tile([1.0, U, 0.0, 0.0, H], (case.im * case.jm, 1)),
+                               rtol=1e-14, atol=1e-14)
+    e2 = np.zeros_like(w, order="F")
+    e2[:, :, 1] = 1.0
+    dz2 = np.zeros_like(w, order="F")
+    O.coeffs_5p_dz2(dz2, w, e2, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], 3,
+                    *a[:9], case.im, case.jm)
+    cp, cv, pr = a[0], a[1], a[2]
+    t = (w[0, 0, 4] - 0.5 * U * U) / cv
+    betas = ph.muref * (ph.tref + ph.cs) / (np.sqrt(ph.tref) * ph.tref)
+    mu = betas / (t + ph.cs) * np.sqrt(t) * t
+    ref = np.array([0.0, -mu, 0.0, 0.0, mu * U * (cp / cv / pr - 1.0)])
+    np.testing.assert_allclose(dz2[3:-3, 3:-3].reshape(-1, 5), np.tile(ref, (case.im * case.jm, 1)), rtol=1e-13,
+                               atol=1e-15)
+
+
+def test_dz_stencil_and_linearity(O):
+    """Dzz is pointwise (block diagonal); Dz at a cell reads only the cross of
+    cells (i +- 1, 2; j +- 1, 2); both are linear in wd."""
+    case = bl_case(14, 10, seed=5)
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    a = case.phys.scheme_args()[:9]
+    rng = np.random.default_rng(8)
+
+    def ops(wd):
+        out = []
+        for fn in (O.coeffs_5p_dz, O.coeffs_5p_dz2):
+            r = np.zeros_like(w, order="F")
+            fn(r, w, np.asfortranarray(wd), g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"],
+               g["volf"], 3, *a, case.im, case.jm)
+            out.append(r)
+        return out
+
+    wd = np.asfortranarray(rng.standard_normal(w.shape))
+    dz, dz2 = ops(wd)
+    ci, cj = 3 + 6, 3 + 5  # interior cell (7, 6) in storage coordinates
+    for (di, dj, cross) in ((1, 0, True), (0, -2, True), (1, 1, False), (3, 0, False), (0, 3, False)):
+        p = np.array(wd, order="F", copy=True)
+        p[ci + di, cj + dj, :] += rng.standard_normal(5)
+        pz, pz2 = ops(p)
+        assert np.array_equal(pz2[ci, cj], dz2[ci, cj])
+        assert np.array_equal(pz[ci, cj], dz[ci, cj]) != cross
+    dzs, dz2s = ops(2.5 * wd)
+    ok, err = close(dzs, 2.5 * dz, 1e-14)
+    assert ok, err
+    ok, err = close(dz2s, 2.5 * dz2, 1e-14)
+    assert ok, err
+
+
+@pytest.mark.skipif(not os.path.exists(oracle.REF_SO), reason="reference library not built here")
+def test_dz_scatter_matches_reference(O):
+    R = oracle.Reference()
+    im, jm, gh = 15, 11, 3
+    nb = im * jm * 49 * 25
+    rng = np.random.default_rng(2)
+    dz = np.asfortranarray(rng.standard_normal((im + 6, jm + 6, 5)))
+    outs = []
+    for lib in (O, R):
+        jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+        for m, l, k in [(0, 0, 0), (2, 5, 3), (4, 6, 6), (1, 3, 0), (3, 2, 5), (0, 4, 4)]:
+            lib.computejacobianfromdz(jac, ia, ja, dz, m, l, k, gh, im, jm)
+        outs.append((jac, ia, ja))
+    for x, y in zip(*outs):
+        np.testing.assert_array_equal(x, y)
