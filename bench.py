@@ -180,7 +180,7 @@ def main():
 
     print(f"bench: {case.name} cells={cells} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
           flush=True)
-    step(first=True)
+    nnz = int(step(first=True)[1].numel())
     for _ in range(max(args.warmup - 1, 0)):
         step()
     torch.cuda.synchronize()
@@ -228,6 +228,10 @@ def main():
         # state + metrics in (96 B/cell) and the 24 x 5 x 5 face-local blocks out
         "k_iface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfi - 2 * (case.im + 1)),
         "k_jface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfj - 3 * case.im),
+        # prune + CSR: the COO values read (count), read again with the CSR
+        # indices and values written (fill)
+        "k_csr_count": 8 * f.nbentry + 4 * 5 * cells,
+        "k_csr_fill": 8 * f.nbentry + 12 * nnz + 4 * 5 * cells,
     }
     dom = max(alg, key=lambda k: kernels[k]["total_ms"])
     avg_dom = kernels[dom]["avg_ms"]

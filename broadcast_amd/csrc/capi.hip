@@ -78,6 +78,7 @@ struct bca_ctx {
   int zone_cap = 0;         // number of zones the three arrays above hold
   int* csr_cnt = nullptr;   // row lengths of the pruned CSR
   int* csr_aux = nullptr;   // scan block sums, total, error flag
+  unsigned long long* csr_mask = nullptr;  // kept masks of the rows in column order
   double* vjpi = nullptr;   // VJP contributions per face and offset, i-faces
   double* vjpj = nullptr;   // j-faces
 };
@@ -304,6 +305,7 @@ int bca_ctx_destroy(bca_ctx* c) {
   if (c->ghostnz) (void)hipFree(c->ghostnz);
   for (int* p : {c->csr_cnt, c->csr_aux})
     if (p) (void)hipFree(p);
+  if (c->csr_mask) (void)hipFree(c->csr_mask);
   delete c;
   return 0;
 }
@@ -614,8 +616,11 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   {
     Timed t(c, "k_assemble_colored");
     dim3 gr = grid2(m.im, m.jm, b);
-    gr.z = step * z.n;
-    k_assemble_colored<<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
+    gr.z = step;
+    if (z.n == 1)
+      k_assemble_colored<1><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
+    else
+      k_assemble_colored<2><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
   }
   BCA_LAUNCHED();
   {
@@ -734,6 +739,7 @@ int bca_csr_colored_count(bca_ctx* c, const double* jac, long nbentry, int nzone
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
   if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
   if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
   if (nb > SCAN_B * SCAN_B) return fail(-1, "too many rows for the scan");
   int* sums = c->csr_aux;
   int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
@@ -746,7 +752,10 @@ int bca_csr_colored_count(bca_ctx* c, const double* jac, long nbentry, int nzone
   gr.z = 5;
   {
     Timed t(c, "k_csr_count");
-    k_csr_rows<false><<<gr, b, 0, s>>>(m, z, jac, mini, c->csr_cnt, nullptr, nullptr, nullptr, bad);
+    if (z.n == 1)
+      k_csr_count<1, const double*><<<gr, b, 0, s>>>(m, z.withjn, jac, mini, c->csr_cnt, c->csr_mask, bad);
+    else
+      k_csr_count<2, const double*><<<gr, b, 0, s>>>(m, z.withjn, jac, mini, c->csr_cnt, c->csr_mask, bad);
   }
   BCA_LAUNCHED();
   k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
@@ -769,15 +778,17 @@ int bca_csr_colored_fill(bca_ctx* c, const double* jac, long nbentry, int nzones
   Zones z;
   int rc = csr_zones(c, nzones, withjn, nbentry, z);
   if (rc) return rc;
-  if (!c->csr_aux) return fail(-1, "bca_csr_colored_count must run first");
+  if (!c->csr_aux || !c->csr_mask) return fail(-1, "bca_csr_colored_count must run first");
   const Mesh& m = c->m;
-  int* bad = c->csr_aux + (long)SCAN_B * SCAN_B + 1;
   const dim3 b(64, 4);
   dim3 gr = grid2(m.im, m.jm, b);
   gr.z = 5;
   {
     Timed t(c, "k_csr_fill");
-    k_csr_rows<true><<<gr, b, 0, c->stream>>>(m, z, jac, mini, nullptr, indptr, indices, data, bad);
+    if (z.n == 1)
+      k_csr_fill<1><<<gr, b, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
+    else
+      k_csr_fill<2><<<gr, b, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
   }
   BCA_LAUNCHED();
   return 0;
@@ -845,14 +856,17 @@ int bca_jbeta_count(bca_ctx* c, const double* jac, const double* jdz, const doub
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
   if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
   if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
   int* sums = c->csr_aux;
   int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
   hipStream_t s = c->stream;
   const dim3 b(64, 4);
   dim3 gr = grid2(m.im, m.jm, b);
   gr.z = 5;
-  k_jbeta_rows<false><<<gr, b, 0, s>>>(m, jac, jdz, jdz2, nullptr, nullptr, 0, mini, c->csr_cnt, nullptr, nullptr,
-                                       nullptr, 0);
+  int* bad = total + 1;
+  k_fill_int<<<1, 64, 0, s>>>(bad, 1, 0);
+  BCA_LAUNCHED();
+  k_csr_count<1, MaxAbs3><<<gr, b, 0, s>>>(m, 0, MaxAbs3{jac, jdz, jdz2}, mini, c->csr_cnt, c->csr_mask, bad);
   BCA_LAUNCHED();
   k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
   BCA_LAUNCHED();
@@ -860,10 +874,11 @@ int bca_jbeta_count(bca_ctx* c, const double* jac, const double* jdz, const doub
   BCA_LAUNCHED();
   k_scan_add<<<nb, SCAN_B, 0, s>>>(indptr, rows, sums, total);
   BCA_LAUNCHED();
-  int host = 0;
-  BCA_HIP(hipMemcpyAsync(&host, total, sizeof(int), hipMemcpyDeviceToHost, s));
+  int host[2] = {0, 0};
+  BCA_HIP(hipMemcpyAsync(host, total, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   BCA_HIP(hipStreamSynchronize(s));
-  *nnz = host;
+  if (host[1]) return fail(-4, "a kept COO entry has a column outside the matrix");
+  *nnz = host[0];
   return 0;
 }
 
@@ -872,6 +887,7 @@ int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const doubl
                    long nnz) {
   if (!c || !jac || !jdz || !jdz2 || !vol || !beta || !indptr || !indices || !vals) return fail(-1, "null argument");
   if (nbeta < 1) return fail(-1, "nbeta must be >= 1");
+  if (!c->csr_aux || !c->csr_mask) return fail(-1, "bca_jbeta_count must run first");
   Zones z;
   int rc = csr_zones(c, 1, 0, nbentry, z);
   if (rc) return rc;
@@ -881,8 +897,8 @@ int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const doubl
   gr.z = 5;
   {
     Timed t(c, "k_jbeta_fill");
-    k_jbeta_rows<true><<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, beta, nbeta, mini, nullptr, indptr, indices,
-                                                vals, nnz);
+    k_jbeta_fill<<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, beta, nbeta, mini, c->csr_mask, indptr, indices,
+                                          vals, nnz);
   }
   BCA_LAUNCHED();
   return 0;

@@ -10,115 +10,197 @@
 // ascending, then valj ascending, then m -- summing the slots that share a
 // column (at most one per zone) in slot order, with no sort of the COO.
 //
-// Two passes over the values: count (row lengths), exclusive scan into
-// indptr, fill.  Reads are coalesced (a wave holds consecutive i of one row
-// kind); each lane writes its row's run.
+// Two passes over the values: count (row lengths and kept masks), exclusive
+// scan into indptr, fill.  Reads are coalesced (a wave holds consecutive i of
+// one row kind and reads the same slot kind); each lane writes its row's run.
 #pragma once
 #include "jacobian.h"
 
 namespace bca {
 
-struct RowWindow {
-  int nvi;             // distinct column indices vali (ascending)
-  int vi[14];
-  unsigned mask[14];   // slots (l + 7 zone) that map to vi[a]
-  int nvj;             // valid valj (ascending) with their k
-  int vj[7], kk[7];
+// The CSR passes read the COO in slot order -- (k, l, zone, m) the same for
+// every lane, so each load is a coalesced row of consecutive i -- and place
+// each kept slot at its column-order index t = (a * 7 + b) * 5 + m, where a
+// is the rank of the slot's vali among the row's distinct values and b that
+// of its valj.  Count: the row's kept set as a bitmask over t (8 words, kept
+// for the fill) and its popcount.  Fill: the slot's CSR position is
+// indptr[row] + popcount of the mask below t; a second slot on the same
+// column (the other zone) is added to the first.
+constexpr int CSR_MASK_WORDS = 8;  // 7 * 2 * 7 * 5 = 490 column-order bits
+
+template <int NZ>
+struct RowOrder {
+  int posl[7 * NZ];  // rank a of the vali of slot (l, zone) among distinct values, -1: dummy
+  int vil[7 * NZ];
+  int posk[7];       // rank b of valj(k), -1: dummy
+  int vjk[7];
 };
 
-__device__ inline void row_window(const Mesh& m, const Zones& z, int i, int j, RowWindow& rw) {
-  const int gh = m.gh, step = 2 * gh + 1;
-  rw.nvi = 0;
-  for (int n = 0; n < z.n; ++n)
-    for (int l = 0; l < step; ++l) {
-      int vali;
-      if (z.n == 1) {
-        if (!seed_vali(i, l, gh, m.im, z.withjn != 0, vali)) continue;
-      } else {
-        bool wrapped;
-        seed_vali_zone(i, l, gh, m.im, n, vali, wrapped);
-      }
-      const unsigned bit = 1u << (l + step * n);
-      int a = 0;
-      while (a < rw.nvi && rw.vi[a] < vali) ++a;
-      if (a < rw.nvi && rw.vi[a] == vali) {
-        rw.mask[a] |= bit;
-        continue;
-      }
-      for (int b = rw.nvi; b > a; --b) {
-        rw.vi[b] = rw.vi[b - 1];
-        rw.mask[b] = rw.mask[b - 1];
-      }
-      rw.vi[a] = vali;
-      rw.mask[a] = bit;
-      ++rw.nvi;
+template <int NZ>
+__device__ inline void row_order(const Mesh& m, int withjn, int i, int j, RowOrder<NZ>& o) {
+  constexpr int NI = 7 * NZ;
+  const int gh = m.gh, im = m.im, jm = m.jm;
+#pragma unroll
+  for (int t = 0; t < NI; ++t) {
+    int v;
+    bool ok = true;
+    if (NZ == 1) {
+      ok = seed_vali(i, t % 7, gh, im, withjn != 0, v);
+    } else {
+      bool wrapped;
+      seed_vali_zone(i, t % 7, gh, im, t / 7, v, wrapped);
     }
-  rw.nvj = 0;
-  for (int k = 0; k < step; ++k) {
-    const int valj = seed_valj(j, k, gh);
-    if (valj >= m.jm) continue;
-    int a = rw.nvj;
-    while (a > 0 && rw.vj[a - 1] > valj) {
-      rw.vj[a] = rw.vj[a - 1];
-      rw.kk[a] = rw.kk[a - 1];
-      --a;
+    o.vil[t] = ok ? v : -1;
+  }
+  // rank among distinct valid values: count of valid values strictly smaller
+  // counted once each (a value shared by two slots counts once)
+#pragma unroll
+  for (int t = 0; t < NI; ++t) {
+    int r = 0;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      bool first = o.vil[u] >= 0;
+#pragma unroll
+      for (int w = 0; w < u; ++w) first = first && o.vil[w] != o.vil[u];
+      r += (first && o.vil[u] < o.vil[t]) ? 1 : 0;
     }
-    rw.vj[a] = valj;
-    rw.kk[a] = k;
-    ++rw.nvj;
+    o.posl[t] = o.vil[t] >= 0 ? r : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int v = seed_valj(j, k, gh);
+    o.vjk[k] = v < jm ? v : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    int r = 0;
+#pragma unroll
+    for (int u = 0; u < 7; ++u) r += (o.vjk[u] >= 0 && o.vjk[u] < o.vjk[k]) ? 1 : 0;
+    o.posk[k] = o.vjk[k] >= 0 ? r : -1;
   }
 }
 
-// Grid over (i, j) x e; FILL = false writes the row length to cnt[row],
-// FILL = true writes the row at indptr[row].  A kept column outside the
-// matrix (possible only for values the reference would also have to
-// reject) raises *bad.
-template <bool FILL>
-__global__ void __launch_bounds__(256) k_csr_rows(Mesh m, Zones z, const double* __restrict__ jac, double mini,
-                                                  int* __restrict__ cnt, const int* __restrict__ indptr,
-                                                  int* __restrict__ indices, double* __restrict__ data,
-                                                  int* __restrict__ bad) {
+__device__ inline void mask_set(unsigned long long (&mk)[CSR_MASK_WORDS], int t) {
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w)
+    if (w == (t >> 6)) mk[w] |= 1ull << (t & 63);
+}
+__device__ inline bool mask_test(const unsigned long long (&mk)[CSR_MASK_WORDS], int t) {
+  bool r = false;
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w)
+    if (w == (t >> 6)) r = (mk[w] >> (t & 63)) & 1ull;
+  return r;
+}
+__device__ inline int mask_rank(const unsigned long long (&mk)[CSR_MASK_WORDS], int t) {
+  int r = 0;
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+    const int lo = w * 64;
+    if (t >= lo + 64) r += __popcll(mk[w]);
+    else if (t > lo) r += __popcll(mk[w] & ((1ull << (t - lo)) - 1ull));
+  }
+  return r;
+}
+
+// Thread index of row (i, j, e) in the mask array: the launch order.
+__device__ inline long csr_tid(const Mesh& m, int i, int j, int e) {
+  return (long)(i - 1) + (long)m.im * ((j - 1) + (long)m.jm * e);
+}
+
+// Visits every non-dummy slot of row (i, j, e) in slot order (zone, l, k, m
+// the same for every lane, so each load is coalesced).
+template <int NZ, class A, class K>
+__device__ inline void row_scan_slots(const Mesh& m, const RowOrder<NZ>& o, const A& jac, int i, int j, int e,
+                                      K&& slot) {
+  const int im = m.im, jm = m.jm;
+  const long plane = (long)im * jm;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)e * plane;
+#pragma unroll
+  for (int n = 0; n < NZ; ++n)
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+      const int a = o.posl[l + 7 * n];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int b = o.posk[k];
+#pragma unroll
+        for (int mm = 0; mm < 5; ++mm) {
+          const long s = base + (long)k * plane * 5 + (long)l * plane * 35 + (long)mm * plane * 245 +
+                         (long)n * plane * 1225;
+          const double v = jac[s];
+          if (a < 0 || b < 0) continue;
+          slot((a * 7 + b) * 5 + mm, mm + 5L * o.vjk[k] + 5L * jm * o.vil[l + 7 * n], v, s);
+        }
+      }
+    }
+}
+
+// Count: kept mask (column order) of every row into masks[tid], length into
+// cnt[row].
+template <int NZ, class A>
+__global__ void __launch_bounds__(256) k_csr_count(Mesh m, int withjn, A jac, double mini, int* __restrict__ cnt,
+                                                   unsigned long long* __restrict__ masks, int* __restrict__ bad) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int e = blockIdx.z;
   if (i > m.im || j > m.jm) return;
-  const int im = m.im, jm = m.jm, step = 2 * m.gh + 1;
-  const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)e * plane;
-  const long ncols = 5L * im * jm;
-  const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-  RowWindow rw;
-  row_window(m, z, i, j, rw);
-  int out = FILL ? indptr[row] : 0;
-  for (int a = 0; a < rw.nvi; ++a)
-    for (int b = 0; b < rw.nvj; ++b)
-      for (int mm = 0; mm < 5; ++mm) {
-        double sum = 0.0;
-        bool any = false;
-        for (int n = 0; n < z.n; ++n)
-          for (int l = 0; l < step; ++l) {
-            if (!(rw.mask[a] >> (l + step * n) & 1u)) continue;
-            const long slot = base + (long)rw.kk[b] * plane * 5 + (long)l * plane * 5 * step +
-                              (long)mm * plane * 5 * step * step + (long)n * plane * 25 * step * step;
-            const double v = jac[slot];
-            if (fabs(v) > mini) {
-              sum = any ? sum + v : v;
-              any = true;
-            }
-          }
-        if (!any) continue;
-        const long col = mm + 5L * rw.vj[b] + 5L * jm * rw.vi[a];
-        if (col < 0 || col >= ncols) {
-          *bad = 1;
-          continue;
-        }
-        if (FILL) {
-          indices[out] = (int)col;
-          data[out] = sum;
-        }
-        ++out;
-      }
-  if (!FILL) cnt[row] = out;
+  RowOrder<NZ> o;
+  row_order<NZ>(m, withjn, i, j, o);
+  unsigned long long mk[CSR_MASK_WORDS];
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) mk[w] = 0;
+  const long ncols = 5L * m.im * m.jm;
+  bool oob = false;
+  row_scan_slots<NZ>(m, o, jac, i, j, e, [&](int t, long col, double v, long) {
+    if (fabs(v) > mini) {
+      mask_set(mk, t);
+      oob = oob || col >= ncols;
+    }
+  });
+  if (oob) *bad = 1;
+  int c = 0;
+  const long tid = csr_tid(m, i, j, e), nt = 5L * m.im * m.jm;
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+    c += __popcll(mk[w]);
+    masks[w * nt + tid] = mk[w];
+  }
+  cnt[e + (j - 1) * 5 + (i - 1) * m.jm * 5] = c;
+}
+
+// Fill: every kept slot at its position; a second slot on a column (the other
+// zone, later in slot order) is added to the first.
+template <int NZ>
+__global__ void __launch_bounds__(256) k_csr_fill(Mesh m, int withjn, const double* __restrict__ jac, double mini,
+                                                  const unsigned long long* __restrict__ masks,
+                                                  const int* __restrict__ indptr, int* __restrict__ indices,
+                                                  double* __restrict__ data) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int e = blockIdx.z;
+  if (i > m.im || j > m.jm) return;
+  RowOrder<NZ> o;
+  row_order<NZ>(m, withjn, i, j, o);
+  unsigned long long mk[CSR_MASK_WORDS], done[CSR_MASK_WORDS];
+  const long tid = csr_tid(m, i, j, e), nt = 5L * m.im * m.jm;
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+    mk[w] = masks[w * nt + tid];
+    done[w] = 0;
+  }
+  const int start = indptr[e + (j - 1) * 5 + (i - 1) * m.jm * 5];
+  row_scan_slots<NZ>(m, o, jac, i, j, e, [&](int t, long col, double v, long) {
+    if (!(fabs(v) > mini)) return;
+    const int pos = start + mask_rank(mk, t);
+    if (NZ > 1 && mask_test(done, t)) {
+      data[pos] = data[pos] + v;
+    } else {
+      indices[pos] = (int)col;
+      data[pos] = v;
+      if (NZ > 1) mask_set(done, t);
+    }
+  });
 }
 
 // ------------------------------------------------------------------ scan

@@ -373,53 +373,49 @@ __global__ void __launch_bounds__(256) k_dz_colored(Mesh m, Phys ph, Metric g, c
 }
 
 // ------------------------------------------------------------------ J(beta)
-// Rows of J(beta) for nb wavenumbers from the three COOs on the Jacobian's
-// slot layout (plain/relaxed single-zone coloring): a slot is kept if any of
-// |jac|, |dz|, |dz2| exceeds mini, each term pruned on its own; values
-// re = jac/vol(row) - beta^2 dz2, im = beta dz, stored [b][nnz] as (re, im).
-template <bool FILL>
-__global__ void __launch_bounds__(256) k_jbeta_rows(Mesh m, const double* __restrict__ jac,
+// J(beta) rows for nb wavenumbers from the three COOs on the Jacobian's slot
+// layout (plain/relaxed single-zone coloring): a slot is kept if any of
+// |jac|, |dz|, |dz2| exceeds mini (k_csr_count over MaxAbs3), each term then
+// pruned on its own; values re = jac/vol(row) - beta^2 dz2, im = beta dz,
+// stored [b][nnz] as (re, im).
+struct MaxAbs3 {
+  const double *a, *b, *c;
+  __device__ double operator[](long s) const { return fmax(fabs(a[s]), fmax(fabs(b[s]), fabs(c[s]))); }
+};
+
+__global__ void __launch_bounds__(256) k_jbeta_fill(Mesh m, const double* __restrict__ jac,
                                                     const double* __restrict__ jdz, const double* __restrict__ jdz2,
                                                     const double* __restrict__ vol, const double* __restrict__ beta,
-                                                    int nb, double mini, int* __restrict__ cnt,
+                                                    int nb, double mini, const unsigned long long* __restrict__ masks,
                                                     const int* __restrict__ indptr, int* __restrict__ indices,
                                                     double* __restrict__ vals, long nnz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int e = blockIdx.z;
   if (i > m.im || j > m.jm) return;
-  const int im = m.im, jm = m.jm, step = 2 * m.gh + 1;
-  const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)e * plane;
-  const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-  const Zones z{1, {1, 0}, {im, 0}, 0};
-  RowWindow rw;
-  row_window(m, z, i, j, rw);
+  RowOrder<1> o;
+  row_order<1>(m, 0, i, j, o);
+  unsigned long long mk[CSR_MASK_WORDS];
+  const long tid = csr_tid(m, i, j, e), nt = 5L * m.im * m.jm;
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) mk[w] = masks[w * nt + tid];
+  const int start = indptr[e + (j - 1) * 5 + (i - 1) * m.jm * 5];
   const double v0 = vol[cidx(m, i, j)];
-  int out = FILL ? indptr[row] : 0;
-  for (int a = 0; a < rw.nvi; ++a)
-    for (int b = 0; b < rw.nvj; ++b)
-      for (int mm = 0; mm < 5; ++mm) {
-        int l = 0;
-        while (!(rw.mask[a] >> l & 1u)) ++l;
-        const long slot =
-            base + (long)rw.kk[b] * plane * 5 + (long)l * plane * 5 * step + (long)mm * plane * 5 * step * step;
-        const double x = jac[slot], y = jdz[slot], y2 = jdz2[slot];
-        const bool kx = fabs(x) > mini, ky = fabs(y) > mini, ky2 = fabs(y2) > mini;
-        if (!(kx || ky || ky2)) continue;
-        if (FILL) {
-          indices[out] = mm + 5 * rw.vj[b] + 5 * jm * rw.vi[a];
-          const double jv = kx ? x / v0 : 0.0, d = ky ? y : 0.0, d2 = ky2 ? y2 : 0.0;
-          for (int q = 0; q < nb; ++q) {
-            const double bq = beta[q];
-            double* o = vals + 2 * ((long)q * nnz + out);
-            o[0] = jv - (bq * bq) * d2;
-            o[1] = bq * d;
-          }
-        }
-        ++out;
-      }
-  if (!FILL) cnt[row] = out;
+  const MaxAbs3 mv{jac, jdz, jdz2};
+  row_scan_slots<1>(m, o, mv, i, j, e, [&](int t, long col, double v, long slot) {
+    if (!(v > mini)) return;
+    const int pos = start + mask_rank(mk, t);
+    indices[pos] = (int)col;
+    const double x = jac[slot], y = jdz[slot], y2 = jdz2[slot];
+    const double jv = fabs(x) > mini ? x / v0 : 0.0;
+    const double d = fabs(y) > mini ? y : 0.0, d2 = fabs(y2) > mini ? y2 : 0.0;
+    for (int q = 0; q < nb; ++q) {
+      const double bq = beta[q];
+      double* out = vals + 2 * ((long)q * nnz + pos);
+      out[0] = jv - (bq * bq) * d2;
+      out[1] = bq * d;
+    }
+  });
 }
 
 }  // namespace bca

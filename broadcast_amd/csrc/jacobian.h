@@ -150,6 +150,7 @@ __device__ inline int color_offset(int k, int j, int n, int gh) {
 // occurs only on the first/last face where one end is a ghost), so this is the
 // whole interior contribution of each seed.  Values -dR/dw (+ coefdiag on the
 // diagonal) go to the reference's COO slots (misc/ComputeJacobian.f90:503-570).
+template <int NZ>
 __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const double* __restrict__ bi,
                                                           const double* __restrict__ bj,
                                                           const double* __restrict__ coefdiag,
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
                                                           int* __restrict__ ja, int write_index) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z % 7, zn = blockIdx.z / 7;
+  const int k = blockIdx.z;
   if (i > m.im || j > m.jm) return;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
   const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
@@ -165,8 +166,10 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
   const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
   const int dj = color_offset(k, j, jm, gh);
   const bool jin = j + dj >= 1 && j + dj <= jm;
-  // wave-uniform plane offsets of the 4 faces for each di (-1: no block)
+  // wave-uniform plane offsets of the 4 faces for each di (-1: no block), and
+  // the zone of the column cell i + di (each cell belongs to one zone)
   long pi0[7], pi1[7], pj0[7], pj1[7];
+  int zc[7];
 #pragma unroll
   for (int D = 0; D < 7; ++D) {
     const int di = D - 3;
@@ -178,22 +181,25 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
     pj0[D] = (jin && o >= 0 && joff_live(m, o, j)) ? (long)o * 25 * nfj + fj0 : -1;
     o = joff_index(di, dj - 1);
     pj1[D] = (jin && o >= 0 && joff_live(m, o, j + 1)) ? (long)o * 25 * nfj + fj1 : -1;
-    if (i + di < z.lo[zn] || i + di > z.hi[zn]) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
+    if (i + di < 1 || i + di > im) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
+    zc[D] = (NZ > 1 && i + di > z.hi[0]) ? 1 : 0;
   }
-  // seed columns of the 7 colors (l, k) of this row
-  int col[7];
-  bool ok[7];
+  // seed columns of the 7 colors (l, k) of this row, per zone
+  int col[NZ][7];
+  bool ok[NZ][7];
   const int rot = (i - 1) % step;
 #pragma unroll
-  for (int l = 0; l < 7; ++l) {
-    int vali, valj;
-    ok[l] = zone_column(m, z, zn, i, j, l, k, vali, valj);
-    col[l] = ok[l] ? valj * 5 + vali * jm * 5 : 0;
-  }
+  for (int zn = 0; zn < NZ; ++zn)
+#pragma unroll
+    for (int l = 0; l < 7; ++l) {
+      int vali, valj;
+      ok[zn][l] = zone_column(m, z, zn, i, j, l, k, vali, valj);
+      col[zn][l] = ok[zn][l] ? valj * 5 + vali * jm * 5 : 0;
+    }
   const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
   const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)zn * plane * 25 * step * step;
-  const int dummy = (z.withjn || z.n > 1) ? 5 * im * jm - 2 : 5 * im * jm - 1;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5;
+  const int dummy = (z.withjn || NZ > 1) ? 5 * im * jm - 2 : 5 * im * jm - 1;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
@@ -218,16 +224,24 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
         r += r < -3 ? 7 : 0;
         r -= r > 3 ? 7 : 0;
         double x = v[0];
+        int xz = zc[0];
 #pragma unroll
-        for (int D = 1; D < 7; ++D) x = (r == D - 3) ? v[D] : x;
-        const long cur = base + (long)e * plane + (long)l * plane * 5 * step + (long)mm * plane * 5 * step * step;
-        const int c = ok[l] ? col[l] + mm : 0;
-        double val = ok[l] ? x : 0.0;
-        if (ok[l] && coefdiag && row == c) val += cd;
-        jac[cur] = val;
-        if (write_index) {
-          ia[cur] = ok[l] ? row : dummy;
-          ja[cur] = c;
+        for (int D = 1; D < 7; ++D) {
+          x = (r == D - 3) ? v[D] : x;
+          xz = (r == D - 3) ? zc[D] : xz;
+        }
+#pragma unroll
+        for (int zn = 0; zn < NZ; ++zn) {
+          const long cur = base + (long)e * plane + (long)l * plane * 5 * step +
+                           (long)mm * plane * 5 * step * step + (long)zn * plane * 25 * step * step;
+          const int c = ok[zn][l] ? col[zn][l] + mm : 0;
+          double val = (ok[zn][l] && xz == zn) ? x : 0.0;
+          if (ok[zn][l] && coefdiag && row == c) val += cd;
+          jac[cur] = val;
+          if (write_index) {
+            ia[cur] = ok[zn][l] ? row : dummy;
+            ja[cur] = c;
+          }
         }
       }
     }
