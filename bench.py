@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--im", type=int, default=512)
     ap.add_argument("--jm", type=int, default=384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Newton (256x128 boundary layer) and J(beta) (512x192, 16 beta) figures")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 512 MiB device-to-device copy first (counter calibration under rocprofv3)")
@@ -270,7 +272,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach_dom / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, case),
             "avg_launch_ms": avg_dom,
             "launches": kernels[dom]["launches"],
             "algorithmic_bytes_per_launch": alg[dom],
@@ -290,9 +292,100 @@ def main():
         },
         "jvp_seed_mcells_per_s": nseeds * cells / (ms_step * 1e-3) / 1e6,
     }
+    if not args.no_extras:
+        out["newton_bl256x128"] = newton_figure()
+        out["jbeta_bl512x192"] = jbeta_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, case):
+    """HBM bytes per launch of `kernel` read by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE on this workload (profiles/hbm_traffic_cyl512x384.json, from the
+    counter runs its summary lists), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_cyl{case.im}x{case.jm}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["traffic"].get(kernel)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def newton_figure():
+    """Plain Newton (no learned guess) on the 256x128 boundary layer (M 0.5,
+    Re_L 1e5, Blasius-type state + 1e-3 noise): wall time to
+    max_e ||R_e||_2 < 1e-10, the per-iteration norms and times."""
+    import torch
+    from broadcast_amd.cases import bl_case
+    from broadcast_amd.flow import Flow, to_device
+    from broadcast_amd.newton import newton
+
+    case = bl_case(256, 128, seed=0)
+    f = Flow(case)
+    w = to_device(case.w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    log = newton(f, w, ite=40, cfl=1e3, tol=1e-10)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    print(f"bench: newton 256x128 {wall:.2f} s, {len(log.norms) - 1} steps, converged {log.converged}",
+          file=sys.stderr, flush=True)
+    return {"seconds_to_1e-10": wall, "converged": log.converged, "newton_steps": len(log.norms) - 1,
+            "norms": log.norms, "seconds_per_iteration": log.seconds, "cfl": 1e3,
+            "start": "Blasius-type state + 1e-3 noise (seed 0), plain Newton"}
+
+
+def jbeta_figure():
+    """J(beta) = Jac/vol - beta^2 Dzz + i beta Dz for beta_k = k/15, k = 0..15
+    on the 512x192 boundary layer: Jacobian, Dz/Dzz and the 16 complex CSR
+    value arrays on the device; HBM figure of the J(beta) fill (the values
+    written, 16 B per complex entry per beta, plus the three COOs read)."""
+    import torch
+    from broadcast_amd import _lib
+    from broadcast_amd.cases import bl_case
+    from broadcast_amd.flow import Flow, to_device
+
+    case = bl_case(512, 192, seed=2000)
+    f = Flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    cd = torch.full((case.im * case.jm,), 1e-2, dtype=torch.float64, device="cuda")
+    betas = [k / 15.0 for k in range(16)]
+    bufs = f.jacobian_buffers()
+    dzb = None
+
+    def run():
+        nonlocal dzb
+        jac, _, _ = f.jacobian_colored(w, coefdiag=cd, bufs=bufs, write_index=dzb is None)
+        jdz, jdz2, ia, ja = f.dz_colored(w, bufs=dzb, write_index=dzb is None)
+        dzb = (jdz, jdz2, ia, ja)
+        return f.jbeta(jac, jdz, jdz2, betas)
+
+    run()
+    torch.cuda.synchronize()
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        indptr, indices, vals = run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    tot, n = kernel_ms(f, "k_jbeta_values")
+    tot_p, n_p = kernel_ms(f, "k_jbeta_parts")
+    dz_tot, dz_n = kernel_ms(f, "k_dz_colored")
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 0), "bca_profile_enable")
+    nnz = indices.numel()
+    fill_ms = tot / max(n, 1)
+    # k_jbeta_values: the 16 complex values written and the 3 components read
+    alg = 16 * len(betas) * nnz + 24 * nnz
+    print(f"bench: J(beta) 512x192 x16 {ms:.2f} ms per batch, values {fill_ms:.3f} ms", file=sys.stderr, flush=True)
+    return {"ms_per_batch": ms, "betas": len(betas), "nnz": nnz, "cells": case.im * case.jm,
+            "k_jbeta_values_ms": fill_ms, "k_jbeta_parts_ms": tot_p / max(n_p, 1),
+            "k_dz_colored_ms": dz_tot / max(dz_n, 1),
+            "k_jbeta_values_gbs": alg / (fill_ms * 1e-3) / 1e9,
+            "k_jbeta_values_frac": alg / (fill_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "workload": "Jacobian + Dz/Dzz seed loops + J(beta) CSR values for 16 wavenumbers"}
 
 
 if __name__ == "__main__":

@@ -1,6 +1,7 @@
 // C ABI of the gfx950 DNC5 path (declared in include/broadcast_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -81,6 +82,8 @@ struct bca_ctx {
   unsigned long long* csr_mask = nullptr;  // kept masks of the rows in column order
   double* vjpi = nullptr;   // VJP contributions per face and offset, i-faces
   double* vjpj = nullptr;   // j-faces
+  double* jbeta_comp = nullptr;  // J(beta) components on the pattern
+  long jbeta_cap = 0;
 };
 
 namespace {
@@ -306,6 +309,7 @@ int bca_ctx_destroy(bca_ctx* c) {
   for (int* p : {c->csr_cnt, c->csr_aux})
     if (p) (void)hipFree(p);
   if (c->csr_mask) (void)hipFree(c->csr_mask);
+  if (c->jbeta_comp) (void)hipFree(c->jbeta_comp);
   delete c;
   return 0;
 }
@@ -891,14 +895,23 @@ int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const doubl
   Zones z;
   int rc = csr_zones(c, 1, 0, nbentry, z);
   if (rc) return rc;
+  if (c->jbeta_cap < nnz) {
+    if ((rc = realloc_planes(c, &c->jbeta_comp, 3 * std::max(nnz, 1L)))) return rc;
+    c->jbeta_cap = nnz;
+  }
   const Mesh& m = c->m;
   const dim3 b(64, 4);
   dim3 gr = grid2(m.im, m.jm, b);
   gr.z = 5;
   {
-    Timed t(c, "k_jbeta_fill");
-    k_jbeta_fill<<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, beta, nbeta, mini, c->csr_mask, indptr, indices,
-                                          vals, nnz);
+    Timed t(c, "k_jbeta_parts");
+    k_jbeta_parts<<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, mini, c->csr_mask, indptr, indices, c->jbeta_comp,
+                                           nnz);
+  }
+  BCA_LAUNCHED();
+  {
+    Timed t(c, "k_jbeta_values");
+    k_jbeta_values<<<grid1(nnz, 256), 256, 0, c->stream>>>(c->jbeta_comp, nnz, beta, nbeta, vals);
   }
   BCA_LAUNCHED();
   return 0;

@@ -383,12 +383,14 @@ struct MaxAbs3 {
   __device__ double operator[](long s) const { return fmax(fabs(a[s]), fmax(fabs(b[s]), fabs(c[s]))); }
 };
 
-__global__ void __launch_bounds__(256) k_jbeta_fill(Mesh m, const double* __restrict__ jac,
-                                                    const double* __restrict__ jdz, const double* __restrict__ jdz2,
-                                                    const double* __restrict__ vol, const double* __restrict__ beta,
-                                                    int nb, double mini, const unsigned long long* __restrict__ masks,
-                                                    const int* __restrict__ indptr, int* __restrict__ indices,
-                                                    double* __restrict__ vals, long nnz) {
+// Pass 1: the column indices and the three pruned components on the pattern,
+// comp[p] = (jac/vol(row), dz, dz2) (zero where pruned).
+__global__ void __launch_bounds__(256) k_jbeta_parts(Mesh m, const double* __restrict__ jac,
+                                                     const double* __restrict__ jdz, const double* __restrict__ jdz2,
+                                                     const double* __restrict__ vol, double mini,
+                                                     const unsigned long long* __restrict__ masks,
+                                                     const int* __restrict__ indptr, int* __restrict__ indices,
+                                                     double* __restrict__ comp, long nnz) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int e = blockIdx.z;
@@ -407,15 +409,26 @@ __global__ void __launch_bounds__(256) k_jbeta_fill(Mesh m, const double* __rest
     const int pos = start + mask_rank(mk, t);
     indices[pos] = (int)col;
     const double x = jac[slot], y = jdz[slot], y2 = jdz2[slot];
-    const double jv = fabs(x) > mini ? x / v0 : 0.0;
-    const double d = fabs(y) > mini ? y : 0.0, d2 = fabs(y2) > mini ? y2 : 0.0;
-    for (int q = 0; q < nb; ++q) {
-      const double bq = beta[q];
-      double* out = vals + 2 * ((long)q * nnz + pos);
-      out[0] = jv - (bq * bq) * d2;
-      out[1] = bq * d;
-    }
+    comp[pos] = fabs(x) > mini ? x / v0 : 0.0;
+    comp[nnz + pos] = fabs(y) > mini ? y : 0.0;
+    comp[2 * nnz + pos] = fabs(y2) > mini ? y2 : 0.0;
   });
+}
+
+// Pass 2, element-wise and coalesced: vals[b][p] = (jv - beta_b^2 dz2, beta_b dz).
+__global__ void __launch_bounds__(256) k_jbeta_values(const double* __restrict__ comp, long nnz,
+                                                      const double* __restrict__ beta, int nb,
+                                                      double* __restrict__ vals) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  const double jv = comp[p], d = comp[nnz + p], d2 = comp[2 * nnz + p];
+  for (int q = 0; q < nb; ++q) {
+    const double bq = beta[q];
+    double2 r;
+    r.x = jv - (bq * bq) * d2;
+    r.y = bq * d;
+    reinterpret_cast<double2*>(vals)[(long)q * nnz + p] = r;
+  }
 }
 
 }  // namespace bca

@@ -22,10 +22,10 @@ by more than `growth`.
 Block-banded LU.  In the reference's unknown order (C-order ravel of
 (im, jm, 5), i outermost) the Jacobian couples i-lines at most 3 apart, so it
 is block banded with blocks of one i-line (5 jm unknowns).  The factorization
-is right-looking block LU: each diagonal block by LU with partial pivoting
-(rocSOLVER getrf through torch.linalg.lu_factor), the off-diagonal blocks by
-triangular solves, the Schur update of the band by one GEMM per step
-(rocBLAS); no fill leaves the band.  Storage is dense per block row,
+is block Gaussian elimination: each diagonal block by LU with partial
+pivoting (rocSOLVER getrf through torch.linalg.lu_factor), the row blocks to
+its right by one solve with it, the band below updated by one GEMM per block
+row (rocBLAS); no fill leaves the band.  Storage is dense per block row,
 R[K] = [A(K, K-kb) ... A(K, K+kb)], (nblocks, B, (2 kb + 1) B) doubles.
 An i-periodic mesh (the O-mesh) couples the first and last i-lines; it is
 solved with j outermost instead (blocks of one j-line), where the band is
@@ -45,13 +45,16 @@ class BlockBandLU:
     """Dense block-banded LU on the device of an n x n CSR matrix whose
     unknown u = e + 5 (j + jm i) (0-based) is permuted to outer-major blocks."""
 
-    def __init__(self, im, jm, outer="i", kb=None):
+    def __init__(self, im, jm, outer="i", kb=None, group=1):
         self.t = dev.torch()
         self.im, self.jm = im, jm
         self.outer = outer
         self.n = 5 * im * jm
-        self.B = 5 * jm if outer == "i" else 5 * im
-        self.nb = im if outer == "i" else jm
+        lines = im if outer == "i" else jm
+        line = 5 * jm if outer == "i" else 5 * im
+        self.B = group * line                      # unknowns per block: `group` outer lines
+        self.nb = -(-lines // group)
+        self.np = self.nb * self.B                 # padded with identity rows past n
         self.kb = kb
 
     def _perm(self, u):
@@ -79,8 +82,14 @@ class BlockBandLU:
             raise ValueError("entry outside the block band")
         R = t.zeros((self.nb, B, W), dtype=data.dtype, device=dv)
         R.view(-1).index_put_(((K * B + rows % B) * W + d * B + cols % B,), data)
+        if self.np > n:
+            pad = t.arange(n, self.np, device=dv)
+            R.view(-1)[((pad // B) * B + pad % B) * W + kb * B + pad % B] = 1.0
         self.R = R
         self.lu, self.piv = [], []
+        # block Gaussian elimination: D_I = A(I, I) after the updates of the
+        # steps before; X(I, J) = D_I^-1 A(I, J) stored in place of A(I, J);
+        # A(K, J) -= A(K, I) X(I, J) for K, J in I+1..I+kb
         for I in range(self.nb):
             LU, piv, _ = t.linalg.lu_factor_ex(R[I, :, kb * B:(kb + 1) * B])
             self.lu.append(LU)
@@ -88,17 +97,11 @@ class BlockBandLU:
             hi = min(I + kb, self.nb - 1)
             if hi == I:
                 continue
-            P, L, U = t.lu_unpack(LU, piv)
-            # U row: L^-1 P^T A(I, J), J = I+1..hi
-            Ur = R[I, :, (kb + 1) * B:(kb + 1 + hi - I) * B]
-            Ur.copy_(t.linalg.solve_triangular(L, P.transpose(0, 1) @ Ur, upper=False, unitriangular=True))
+            Xr = R[I, :, (kb + 1) * B:(kb + 1 + hi - I) * B]
+            Xr.copy_(t.linalg.lu_solve(LU, piv, Xr))
             for dd in range(1, hi - I + 1):
                 K2 = I + dd
-                Lb = R[K2, :, (kb - dd) * B:(kb - dd + 1) * B]
-                # L column: A(K, I) U^-1, then the Schur update of row K
-                Lb.copy_(t.linalg.solve_triangular(U, Lb, upper=True, left=False))
-                tgt = R[K2, :, (kb - dd + 1) * B:(kb - dd + 1 + hi - I) * B]
-                tgt.sub_(Lb @ Ur)
+                R[K2, :, (kb - dd + 1) * B:(kb - dd + 1 + hi - I) * B].sub_(R[K2, :, (kb - dd) * B:(kb - dd + 1) * B] @ Xr)
         return self
 
     def solve(self, b):
@@ -106,23 +109,21 @@ class BlockBandLU:
         t = self.t
         B, kb, nb = self.B, self.kb, self.nb
         if self.outer == "i":
-            y = b.reshape(nb, B).clone()
+            y = b.reshape(-1)
         else:
-            y = b.reshape(self.im, self.jm, 5).permute(1, 0, 2).reshape(nb, B).clone()
+            y = b.reshape(self.im, self.jm, 5).permute(1, 0, 2).reshape(-1)
+        y = t.cat((y, y.new_zeros(self.np - self.n))).reshape(nb, B)
         R = self.R
         for I in range(nb):
-            for dd in range(1, min(kb, I) + 1):
-                y[I] -= R[I, :, (kb - dd) * B:(kb - dd + 1) * B] @ y[I - dd]
-            P, L, _ = t.lu_unpack(self.lu[I], self.piv[I])
-            y[I] = t.linalg.solve_triangular(L, (P.transpose(0, 1) @ y[I]).unsqueeze(1), upper=False,
-                                             unitriangular=True).squeeze(1)
+            y[I] = t.linalg.lu_solve(self.lu[I], self.piv[I], y[I].unsqueeze(1)).squeeze(1)
+            for dd in range(1, min(kb, nb - 1 - I) + 1):
+                y[I + dd] -= R[I + dd, :, (kb - dd) * B:(kb - dd + 1) * B] @ y[I]
         for I in range(nb - 1, -1, -1):
             for dd in range(1, min(kb, nb - 1 - I) + 1):
                 y[I] -= R[I, :, (kb + dd) * B:(kb + dd + 1) * B] @ y[I + dd]
-            U = t.triu(self.lu[I])
-            y[I] = t.linalg.solve_triangular(U, y[I].unsqueeze(1), upper=True).squeeze(1)
+        y = y.reshape(-1)[:self.n]
         if self.outer == "i":
-            return y.reshape(-1)
+            return y
         return y.reshape(self.jm, self.im, 5).permute(1, 0, 2).reshape(-1)
 
 
@@ -141,7 +142,7 @@ def interior(flow, a):
     return a.view(5, flow.nj, flow.ni)[:, 3:-3, 3:-3].permute(2, 1, 0).contiguous()
 
 
-def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth=1e3, cfl_cut=10.0):
+def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth=1e3, cfl_cut=10.0, group=1):
     """The reference's Newton iteration on the flat device state w (updated
     in place).  Returns a NewtonLog."""
     t = dev.torch()
@@ -192,7 +193,7 @@ def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth
         jac, ia, ja = flow.jacobian_colored(w, coefdiag=coefdiag, bufs=bufs, write_index=first)
         first = False
         indptr, indices, data = flow.csr(jac)
-        lu = BlockBandLU(flow.im, flow.jm, outer=outer).factor(indptr, indices, data)
+        lu = BlockBandLU(flow.im, flow.jm, outer=outer, group=group).factor(indptr, indices, data)
         dw = lu.solve(interior(flow, res).reshape(-1))
         w_prev, l2_prev = w.clone(), l2
         wi = w.view(5, flow.nj, flow.ni)[:, 3:-3, 3:-3]

@@ -15,10 +15,11 @@ def O():
     return oracle.Oracle()
 
 
-@pytest.mark.parametrize("mk,outer", [(lambda: bl_case(16, 10, seed=21), "i"),
-                                      (lambda: bl_case(12, 9, seed=22, reynolds=3e5), "j"),
-                                      (lambda: cylinder_case(14, 9, seed=23), "j")])
-def test_block_band_lu_matches_sparse_solve(O, mk, outer):
+@pytest.mark.parametrize("mk,outer,group", [(lambda: bl_case(16, 10, seed=21), "i", 1),
+                                            (lambda: bl_case(16, 10, seed=21), "i", 3),
+                                            (lambda: bl_case(12, 9, seed=22, reynolds=3e5), "j", 1),
+                                            (lambda: cylinder_case(14, 9, seed=23), "j", 2)])
+def test_block_band_lu_matches_sparse_solve(O, mk, outer, group):
     import scipy.sparse.linalg as spla
     import torch
     from broadcast_amd.newton import BlockBandLU
@@ -33,7 +34,7 @@ def test_block_band_lu_matches_sparse_solve(O, mk, outer):
     J.sort_indices()
     b = rng.standard_normal(J.shape[0])
     ref = spla.spsolve(J.tocsc(), b)
-    lu = BlockBandLU(case.im, case.jm, outer=outer).factor(torch.from_numpy(J.indptr.astype(np.int32)),
+    lu = BlockBandLU(case.im, case.jm, outer=outer, group=group).factor(torch.from_numpy(J.indptr.astype(np.int32)),
                                                            torch.from_numpy(J.indices.astype(np.int32)),
                                                            torch.from_numpy(J.data.copy()))
     x = lu.solve(torch.from_numpy(b)).numpy()
@@ -44,4 +45,4 @@ def test_block_band_lu_matches_sparse_solve(O, mk, outer):
     berr_ref = np.linalg.norm(J @ ref - b, np.inf) / (anorm * np.linalg.norm(ref, np.inf) + np.linalg.norm(b, np.inf))
     assert berr <= max(10 * berr_ref, 1e-15), (berr, berr_ref)
     assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref)
-    assert lu.kb <= (4 if outer == "j" else 3)
+    assert lu.kb <= -(-(4 if outer == "j" else 3) // group)
