@@ -784,15 +784,13 @@ int bca_csr_colored_fill(bca_ctx* c, const double* jac, long nbentry, int nzones
   if (rc) return rc;
   if (!c->csr_aux || !c->csr_mask) return fail(-1, "bca_csr_colored_count must run first");
   const Mesh& m = c->m;
-  const dim3 b(64, 4);
-  dim3 gr = grid2(m.im, m.jm, b);
-  gr.z = 5;
+  const dim3 gr((m.im + FILL_TI - 1) / FILL_TI, m.jm, 5);
   {
     Timed t(c, "k_csr_fill");
     if (z.n == 1)
-      k_csr_fill<1><<<gr, b, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
+      k_csr_fill<1><<<gr, FILL_THREADS, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
     else
-      k_csr_fill<2><<<gr, b, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
+      k_csr_fill<2><<<gr, FILL_THREADS, 0, c->stream>>>(m, z.withjn, jac, mini, c->csr_mask, indptr, indices, data);
   }
   BCA_LAUNCHED();
   return 0;

@@ -169,38 +169,116 @@ __global__ void __launch_bounds__(256) k_csr_count(Mesh m, int withjn, A jac, do
   cnt[e + (j - 1) * 5 + (i - 1) * m.jm * 5] = c;
 }
 
-// Fill: every kept slot at its position; a second slot on a column (the other
-// zone, later in slot order) is added to the first.
+// Fill.  A block owns FILL_TI consecutive i of one (j, e): it first copies
+// every slot value of those rows into LDS (a 128-byte run of consecutive i
+// per slot plane, so the COO is read once, in whole lines), then each wave
+// writes whole rows in CSR order: lane L of pass w takes column-order index
+// t = 64 w + L, tests the kept mask and stores at indptr[row] + the popcount
+// of the mask below t, so a pass writes one contiguous run of the row.  The
+// value of column t sums the kept slots on it (at most one per zone: within
+// a zone the 7 window columns of a row are distinct mod 7), zone 0 first --
+// the order scipy's sum of duplicates sees them in the COO.
+constexpr int FILL_TI = 16;        // rows (consecutive i) per block
+constexpr int FILL_THREADS = 512;  // 8 waves, FILL_TI / 8 rows each
+
 template <int NZ>
-__global__ void __launch_bounds__(256) k_csr_fill(Mesh m, int withjn, const double* __restrict__ jac, double mini,
-                                                  const unsigned long long* __restrict__ masks,
-                                                  const int* __restrict__ indptr, int* __restrict__ indices,
-                                                  double* __restrict__ data) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int e = blockIdx.z;
-  if (i > m.im || j > m.jm) return;
-  RowOrder<NZ> o;
-  row_order<NZ>(m, withjn, i, j, o);
-  unsigned long long mk[CSR_MASK_WORDS], done[CSR_MASK_WORDS];
-  const long tid = csr_tid(m, i, j, e), nt = 5L * m.im * m.jm;
+__global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, const double* __restrict__ jac,
+                                                          double mini, const unsigned long long* __restrict__ masks,
+                                                          const int* __restrict__ indptr, int* __restrict__ indices,
+                                                          double* __restrict__ data) {
+  constexpr int NS = 245 * NZ;          // slots per row: (k, l, m, zone)
+  constexpr int NA = 7 * NZ;            // distinct window columns in i, at most
+  constexpr int PITCH = FILL_TI + 1;    // odd pitch: lanes reading different slots spread over the banks
+  __shared__ double tile[NS * PITCH];
+  __shared__ signed char inv_l[FILL_TI][NZ][NA];  // slot l of zone n on column rank a, -1: none
+  __shared__ int col_a[FILL_TI][NA];              // 5 jm vali of column rank a
+  __shared__ signed char inv_k[7];                // slot k on column rank b
+  __shared__ int col_b[7];                        // 5 valj of column rank b
+  __shared__ unsigned long long mk_s[CSR_MASK_WORDS][FILL_TI];
+  __shared__ int start_s[FILL_TI];
+  const int tid = threadIdx.x;
+  const int i0 = 1 + blockIdx.x * FILL_TI, j = 1 + blockIdx.y, e = blockIdx.z;
+  const int im = m.im, jm = m.jm;
+  const long plane = (long)im * jm;
+  const long nt = 5L * im * jm;
+  {
+    // slot values: thread (s0, il) loads planes s0, s0 + 32, ...
+    const int il = tid % FILL_TI, s0 = tid / FILL_TI;
+    const bool in = i0 + il <= im;
+    const double* src = jac + (i0 - 1 + il) + (long)(j - 1) * im + (long)e * plane;
+    constexpr int STEP = FILL_THREADS / FILL_TI;
+    constexpr int NL = (NS + STEP - 1) / STEP;
+    double v[NL];
 #pragma unroll
-  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
-    mk[w] = masks[w * nt + tid];
-    done[w] = 0;
-  }
-  const int start = indptr[e + (j - 1) * 5 + (i - 1) * m.jm * 5];
-  row_scan_slots<NZ>(m, o, jac, i, j, e, [&](int t, long col, double v, long) {
-    if (!(fabs(v) > mini)) return;
-    const int pos = start + mask_rank(mk, t);
-    if (NZ > 1 && mask_test(done, t)) {
-      data[pos] = data[pos] + v;
-    } else {
-      indices[pos] = (int)col;
-      data[pos] = v;
-      if (NZ > 1) mask_set(done, t);
+    for (int q = 0; q < NL; ++q) {
+      const int s = s0 + q * STEP;
+      v[q] = (in && s < NS) ? src[5L * s * plane] : 0.0;
     }
-  });
+    if (tid < FILL_TI) {
+      const int i = i0 + tid;
+      if (i <= im) {
+        RowOrder<NZ> o;
+        row_order<NZ>(m, withjn, i, j, o);
+#pragma unroll
+        for (int n = 0; n < NZ; ++n)
+#pragma unroll
+          for (int a = 0; a < NA; ++a) inv_l[tid][n][a] = -1;
+#pragma unroll
+        for (int t = 0; t < NA; ++t)
+          if (o.posl[t] >= 0) {
+            inv_l[tid][t / 7][o.posl[t]] = (signed char)(t % 7);
+            col_a[tid][o.posl[t]] = 5 * jm * o.vil[t];
+          }
+        if (tid == 0) {
+#pragma unroll
+          for (int k = 0; k < 7; ++k)
+            if (o.posk[k] >= 0) {
+              inv_k[o.posk[k]] = (signed char)k;
+              col_b[o.posk[k]] = 5 * o.vjk[k];
+            }
+        }
+        start_s[tid] = indptr[e + (j - 1) * 5 + (i - 1) * jm * 5];
+      }
+    }
+    if (tid < CSR_MASK_WORDS * FILL_TI) {
+      const int w = tid / FILL_TI, r = tid % FILL_TI;
+      if (i0 + r <= im) mk_s[w][r] = masks[w * nt + csr_tid(m, i0 + r, j, e)];
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int s = s0 + q * STEP;
+      if (s < NS) tile[s * PITCH + il] = v[q];
+    }
+  }
+  __syncthreads();
+  const int wave = tid / 64, lane = tid % 64;
+  for (int r = wave; r < FILL_TI; r += FILL_THREADS / 64) {
+    if (i0 + r > im) break;
+    int pos = start_s[r];
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+      const unsigned long long mk = mk_s[w][r];
+      if (mk == 0) continue;
+      if ((mk >> lane) & 1ull) {
+        const int t = w * 64 + lane;
+        const int a = t / 35, b = (t / 5) % 7, mm = t % 5;
+        const int k = inv_k[b];
+        double val = 0.0;
+#pragma unroll
+        for (int n = 0; n < NZ; ++n) {
+          const int l = inv_l[r][n][a];
+          if (l >= 0) {
+            const double x = tile[(k + 7 * l + 49 * mm + 245 * n) * PITCH + r];
+            if (fabs(x) > mini) val += x;
+          }
+        }
+        const int p = pos + __popcll(mk & ((1ull << lane) - 1ull));
+        indices[p] = mm + col_b[b] + col_a[r][a];
+        data[p] = val;
+      }
+      pos += __popcll(mk);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ scan
