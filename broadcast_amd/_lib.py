@@ -31,6 +31,11 @@ class BcOp(C.Structure):
 
 BC_INLET, BC_NOREF, BC_EXTRAP2, BC_WALL_ADIA, BC_JOIN = range(5)
 
+
+class BatchItem(C.Structure):
+    _fields_ = [("im", C.c_int), ("jm", C.c_int), ("g", Metrics), ("phys", Phys), ("ops", C.POINTER(BcOp)),
+                ("nops", C.c_int), ("w", _dp), ("wd", _dp), ("residu", _dp), ("residud", _dp)]
+
 _SIGS = {
     "bca_ctx_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, _dp],
     "bca_ctx_destroy": [_dp],
@@ -69,6 +74,10 @@ _SIGS = {
     "bca_jbeta_fill": [_dp, _dp, _dp, _dp, C.c_long, C.c_double, _dp, _dp, C.c_int, _dp, _dp, _dp, C.c_long],
     "bca_jacobian_colored_2zones": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
                                     C.c_int, _dp, _dp, _dp, C.c_long, C.c_int],
+    "bca_batch_create": [C.POINTER(C.c_void_p), _dp],
+    "bca_batch_set_stream": [_dp, _dp],
+    "bca_batch_destroy": [_dp],
+    "bca_batch_residual_jvp": [_dp, C.POINTER(BatchItem), C.c_int],
 }
 
 # Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).

@@ -243,6 +243,19 @@ def bl_case(im: int, jm: int, *, mach=0.5, reynolds=1e5, x_le=0.1, length=1.0,
                 field, wbd, interf)
 
 
+def uneven_bl_cases(n: int = 8, seed: int = 2000, im_range=(128, 1024), jm_range=(64, 256)):
+    """Boundary-layer meshes of log-uniform random size: instance k draws
+    im in im_range and jm in jm_range log-uniformly from seed + k and uses
+    seed + k for its state noise too."""
+    out = []
+    for k in range(n):
+        rng = np.random.default_rng(seed + k)
+        im = int(round(np.exp(rng.uniform(np.log(im_range[0]), np.log(im_range[1])))))
+        jm = int(round(np.exp(rng.uniform(np.log(jm_range[0]), np.log(jm_range[1])))))
+        out.append(bl_case(im, jm, seed=seed + k))
+    return out
+
+
 def cylinder_case(im: int, jm: int, *, mach=0.1, reynolds=50.0, r0=0.5,
                   r1=100.0, dr1=5e-3, noise=1e-4, seed=0, k2=0.0, k4=1.0,
                   name=None) -> Case:

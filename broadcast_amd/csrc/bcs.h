@@ -46,13 +46,10 @@ struct StateView {
 // the Roe-state pressure lacks the 1/2 (:49), velo2 uses ww unsquared (:69),
 // the ghost spanwise momentum is field(j, de, 4) (:160).
 template <class T>
-__global__ void k_bc_inlet(Mesh m, T* __restrict__ wbase, int loc, Box b, const double* __restrict__ field,
-                           int lm, const double* __restrict__ nxp, const double* __restrict__ nyp,
-                           double gam, long cstride) {
-  T* wp = wbase + (long)blockIdx.y * cstride;
+__device__ inline void bc_inlet_line(const Mesh& m, T* __restrict__ wp, int loc, Box b,
+                                     const double* __restrict__ field, int lm, const double* __restrict__ nxp,
+                                     const double* __restrict__ nyp, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (l > s.lmax) return;
   StateView<T> w{m, wp};
   auto F = [&](int a, int d, int e) { return field[(a - 1) + (long)lm * ((d - 1) + (long)m.gh * e)]; };
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
@@ -139,15 +136,21 @@ __global__ void k_bc_inlet(Mesh m, T* __restrict__ wbase, int loc, Box b, const 
   }
 }
 
+template <class T>
+__global__ void k_bc_inlet(Mesh m, T* __restrict__ wbase, int loc, Box b, const double* __restrict__ field,
+                           int lm, const double* __restrict__ nxp, const double* __restrict__ nyp,
+                           double gam, long cstride) {
+  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
+    bc_inlet_line(m, wbase + (long)blockIdx.y * cstride, loc, b, field, lm, nxp, nyp, gam, l);
+}
+
 // Non-reflecting far field against the reference state wbd(l, :).
 template <class T>
-__global__ void k_bc_noref(Mesh m, T* __restrict__ wbase, const double* __restrict__ wbd, int lm, int loc,
-                           Box b, const double* __restrict__ nxp, const double* __restrict__ nyp,
-                           double gam, long cstride) {
-  T* wp = wbase + (long)blockIdx.y * cstride;
+__device__ inline void bc_noref_line(const Mesh& m, T* __restrict__ wp, const double* __restrict__ wbd, int lm,
+                                     int loc, Box b, const double* __restrict__ nxp,
+                                     const double* __restrict__ nyp, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (l > s.lmax) return;
   StateView<T> w{m, wp};
   const int i1 = s.i0 * s.i0, j1 = s.j0 * s.j0;
   const int i = b.imin + (l - 1) * j1;
@@ -206,11 +209,17 @@ __global__ void k_bc_noref(Mesh m, T* __restrict__ wbase, const double* __restri
 }
 
 template <class T>
-__global__ void k_bc_extrap2(Mesh m, T* __restrict__ wbase, int loc, Box b, int em, long cstride) {
-  T* wp = wbase + (long)blockIdx.y * cstride;
-  const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
+__global__ void k_bc_noref(Mesh m, T* __restrict__ wbase, const double* __restrict__ wbd, int lm, int loc,
+                           Box b, const double* __restrict__ nxp, const double* __restrict__ nyp,
+                           double gam, long cstride) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (l > s.lmax) return;
+  if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
+    bc_noref_line(m, wbase + (long)blockIdx.y * cstride, wbd, lm, loc, b, nxp, nyp, gam, l);
+}
+
+template <class T>
+__device__ inline void bc_extrap2_line(const Mesh& m, T* __restrict__ wp, int loc, Box b, int em, int l) {
+  const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   StateView<T> w{m, wp};
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
   const int j = b.jmin + (l - 1) * s.i0 * s.i0;
@@ -220,15 +229,19 @@ __global__ void k_bc_extrap2(Mesh m, T* __restrict__ wbase, int loc, Box b, int 
           2.0 * w(i - s.i0 * (de - 1), j - s.j0 * (de - 1), e) - w(i - s.i0 * (de - 2), j - s.j0 * (de - 2), e);
 }
 
+template <class T>
+__global__ void k_bc_extrap2(Mesh m, T* __restrict__ wbase, int loc, Box b, int em, long cstride) {
+  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
+    bc_extrap2_line(m, wbase + (long)blockIdx.y * cstride, loc, b, em, l);
+}
+
 // Adiabatic no-slip wall: first ghost from the 2nd-order zero-normal-gradient
 // pressure and the isentropic density, the deeper ones mirror interior cells
 // with reversed velocity.
 template <class T>
-__global__ void k_bc_wall(Mesh m, T* __restrict__ wbase, int loc, Box b, double gam, long cstride) {
-  T* wp = wbase + (long)blockIdx.y * cstride;
+__device__ inline void bc_wall_line(const Mesh& m, T* __restrict__ wp, int loc, Box b, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (l > s.lmax) return;
   StateView<T> w{m, wp};
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
   const int j = b.jmin + (l - 1) * s.i0 * s.i0;
@@ -263,20 +276,30 @@ __global__ void k_bc_wall(Mesh m, T* __restrict__ wbase, int loc, Box b, double 
   }
 }
 
+template <class T>
+__global__ void k_bc_wall(Mesh m, T* __restrict__ wbase, int loc, Box b, double gam, long cstride) {
+  const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
+    bc_wall_line(m, wbase + (long)blockIdx.y * cstride, loc, b, gam, l);
+}
+
 // Block copy between two boxes of one block (the periodic join of the O-mesh).
 // prr/prd: {imin, jmin, imax, jmax}; tr: the index transform (tr[0] axis and
 // sign of the receiving i, tr[1] of j).  em components.
+__host__ __device__ inline int join_count(Box prd, int tr0, int tr1) {
+  const int idir = tr0 >= 0 ? tr0 : -tr0, jdir = tr1 >= 0 ? tr1 : -tr1;
+  const int plo[2] = {prd.imin, prd.jmin}, phi[2] = {prd.imax, prd.jmax};
+  return (phi[idir - 1] - plo[idir - 1] + 1) * (phi[jdir - 1] - plo[jdir - 1] + 1);
+}
+
 template <class T>
-__global__ void k_join(Mesh m, T* __restrict__ wbase, Box prr, Box prd, int tr0, int tr1, int em,
-                       int ni_arr, int nj_arr, long plane, long cstride) {
-  T* wp = wbase + (long)blockIdx.y * cstride;
+__device__ inline void join_cell(const Mesh& m, T* __restrict__ wp, Box prr, Box prd, int tr0, int tr1, int em,
+                                 int ni_arr, long plane, int t) {
   const int istep = tr0 >= 0 ? 1 : -1, jstep = tr1 >= 0 ? 1 : -1;
   const int idir = tr0 >= 0 ? tr0 : -tr0, jdir = tr1 >= 0 ? tr1 : -tr1;
   const int plo[2] = {prd.imin, prd.jmin}, phi[2] = {prd.imax, prd.jmax};
   const int ni_ = phi[idir - 1] - plo[idir - 1] + 1;
   const int nj_ = phi[jdir - 1] - plo[jdir - 1] + 1;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ni_ * nj_) return;
   const int a = t % ni_, bq = t / ni_;          // receiving offsets (idd-1, jdd-1)
   const int ii = istep == 1 ? 1 + a : ni_ - a;  // loop counters of the donor walk
   const int jj = jstep == 1 ? 1 + bq : nj_ - bq;
@@ -286,6 +309,14 @@ __global__ void k_join(Mesh m, T* __restrict__ wbase, Box prr, Box prd, int tr0,
   const long rdst = (ir + off) + (long)ni_arr * (jr + off);
   const long rsrc = (id + off) + (long)ni_arr * (jd + off);
   for (int e = 0; e < em; ++e) wp[rdst + e * plane] = wp[rsrc + e * plane];
+}
+
+template <class T>
+__global__ void k_join(Mesh m, T* __restrict__ wbase, Box prr, Box prd, int tr0, int tr1, int em,
+                       int ni_arr, int nj_arr, long plane, long cstride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < join_count(prd, tr0, tr1))
+    join_cell(m, wbase + (long)blockIdx.y * cstride, prr, prd, tr0, tr1, em, ni_arr, plane, t);
 }
 
 }  // namespace bca

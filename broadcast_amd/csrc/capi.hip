@@ -18,6 +18,7 @@
 #include "vjp.h"
 #include "csr.h"
 #include "dz.h"
+#include "batch.h"
 
 using namespace bca;
 
@@ -224,11 +225,9 @@ int box_ok(const Mesh& m, int loc, const int* bx) {
   return 0;
 }
 
-template <class T>
-int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny, int ncolor = 1,
-              long cstride = 0) {
-  const Mesh& m = c->m;
-  hipStream_t s = c->stream;
+// Shape checks of one BC operation against the mesh (the kernels index w,
+// field, wbd and the normals from the boxes without further checks).
+int bc_check(const Mesh& m, const bca_bc_op& op, const double* nx, const double* ny) {
   const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
   int rc = 0;
   if (op.kind != BCA_BC_JOIN && (rc = box_ok(m, op.loc, op.box))) return rc;
@@ -240,21 +239,14 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
       // field(l, de, :): both must stay inside field(lm, gh, 5)
       if (b.imin < 1 || b.imax > m.gh || b.jmin < 1 || b.jmax > op.lm || sd.lmax > op.lm)
         return fail(-1, "inlet interface does not fit the field(lm, gh, 5) profile");
-      k_bc_inlet<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny,
-                                                                   op.gam, cstride);
-      break;
+      return 0;
     case BCA_BC_NOREF:
       if (!op.data || !nx || !ny) return fail(-1, "noref needs wbd, nx, ny");
       if (op.lm < sd.lmax) return fail(-1, "wbd shorter than the interface");
-      k_bc_noref<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny,
-                                                                   op.gam, cstride);
-      break;
+      return 0;
     case BCA_BC_EXTRAP2:
-      k_bc_extrap2<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, 5, cstride);
-      break;
     case BCA_BC_WALL_ADIA:
-      k_bc_wall<T><<<dim3(grid1(sd.lmax, 64), ncolor), 64, 0, s>>>(m, w, op.loc, b, op.gam, cstride);
-      break;
+      return 0;
     case BCA_BC_JOIN: {
       const Box prr{op.box[0], op.box[1], op.box[2], op.box[3]};
       const Box prd{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
@@ -266,13 +258,49 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
       if (op.tr[0] != 1 || op.tr[1] != 2) return fail(-1, "batched join supports the identity transform only");
       if (prr.imax - prr.imin != prd.imax - prd.imin || prr.jmax - prr.jmin != prd.jmax - prd.jmin)
         return fail(-1, "join boxes differ in size");
-      const long n = (long)(prd.imax - prd.imin + 1) * (prd.jmax - prd.jmin + 1);
-      k_join<T><<<dim3(grid1(n, 256), ncolor), 256, 0, s>>>(m, w, prr, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj,
-                                                             m.ncell, cstride);
-      break;
+      return 0;
     }
     default:
       return fail(-1, "unknown boundary-condition kind");
+  }
+}
+
+// work items of a checked BC operation: lines of the interface, or cells of
+// the join box
+long bc_items(const bca_bc_op& op) {
+  const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
+  if (op.kind == BCA_BC_JOIN) return join_count(Box{op.box2[0], op.box2[1], op.box2[2], op.box2[3]}, 1, 2);
+  return decode_side(op.loc, b.imin, b.jmin, b.imax, b.jmax).lmax;
+}
+
+template <class T>
+int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny, int ncolor = 1,
+              long cstride = 0) {
+  const Mesh& m = c->m;
+  hipStream_t s = c->stream;
+  const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
+  int rc = bc_check(m, op, nx, ny);
+  if (rc) return rc;
+  const unsigned nb = grid1(bc_items(op), op.kind == BCA_BC_JOIN ? 256 : 64);
+  switch (op.kind) {
+    case BCA_BC_INLET:
+      k_bc_inlet<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny, op.gam, cstride);
+      break;
+    case BCA_BC_NOREF:
+      k_bc_noref<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny, op.gam, cstride);
+      break;
+    case BCA_BC_EXTRAP2:
+      k_bc_extrap2<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, 5, cstride);
+      break;
+    case BCA_BC_WALL_ADIA:
+      k_bc_wall<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.gam, cstride);
+      break;
+    default: {
+      const Box prd{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
+      k_join<T><<<dim3(nb, ncolor), 256, 0, s>>>(m, w, b, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj, m.ncell,
+                                                  cstride);
+      break;
+    }
   }
   BCA_LAUNCHED();
   return 0;
@@ -913,6 +941,153 @@ int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const doubl
   }
   BCA_LAUNCHED();
   return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+struct bca_batch {
+  hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // per-mesh planes of the scalar type
+  long scratch_bytes = 0;
+  void* table = nullptr;    // device copy of the items and the plan
+  void* host = nullptr;     // pinned staging of the table
+  long table_bytes = 0;
+  hipEvent_t copied = nullptr;  // the last table copy (the staging is reused after it)
+  bool pending = false;
+};
+
+namespace {
+
+int batch_grow(void** p, long* have, long want, bool pinned) {
+  if (*have >= want) return 0;
+  if (*p) (void)(pinned ? hipHostFree(*p) : hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  if (pinned) BCA_HIP(hipHostMalloc(p, want, hipHostMallocDefault));
+  else BCA_HIP(hipMalloc(p, want));
+  *have = want;
+  return 0;
+}
+
+template <class T>
+int run_batch(bca_batch* b, const bca_batch_item* items, int n) {
+  if (n < 1 || n > BATCH_MAX) return fail(-1, "batch size must be 1.." + std::to_string(BATCH_MAX));
+  // scratch layout and checks
+  long planes = 0;
+  std::vector<Mesh> ms(n);
+  for (int k = 0; k < n; ++k) {
+    const bca_batch_item& it = items[k];
+    if (!it.w || !it.residu || !it.g.nx || !it.g.ny || !it.g.vol || !it.g.volf)
+      return fail(-1, "batch item " + std::to_string(k) + ": null argument");
+    if (std::is_same<T, double>::value != (it.wd == nullptr) || (it.wd && !it.residud))
+      return fail(-1, "batch items must all carry wd and residud, or none");
+    if (it.nops < 0 || it.nops > BATCH_OPS || (it.nops && !it.ops))
+      return fail(-1, "batch item " + std::to_string(k) + ": 0.." + std::to_string(BATCH_OPS) + " BC operations");
+    ms[k] = make_mesh(it.im, it.jm, 3);
+    int rc = check_mesh(ms[k]);
+    if (rc) return rc;
+    for (int o = 0; o < it.nops; ++o)
+      if ((rc = bc_check(ms[k], it.ops[o], it.g.nx, it.g.ny))) return rc;
+    planes += (5 + NPRIM + NGRAD) * ms[k].ncell + 10 * ms[k].nnode;
+  }
+  int rc = batch_grow(&b->scratch, &b->scratch_bytes, planes * (long)sizeof(T), false);
+  if (rc) return rc;
+  const long tbytes = (long)sizeof(BatchItem<T>) * BATCH_MAX + (long)sizeof(BatchPlan);
+  if (b->pending) {
+    BCA_HIP(hipEventSynchronize(b->copied));
+    b->pending = false;
+  }
+  long have = b->table_bytes;
+  if ((rc = batch_grow(&b->table, &b->table_bytes, tbytes, false))) return rc;
+  if ((rc = batch_grow(&b->host, &have, tbytes, true))) return rc;
+  BatchItem<T>* hi = reinterpret_cast<BatchItem<T>*>(b->host);
+  BatchPlan* plan = reinterpret_cast<BatchPlan*>(hi + BATCH_MAX);
+  std::memset(b->host, 0, tbytes);
+  T* p = reinterpret_cast<T*>(b->scratch);
+  plan->n = n;
+  plan->maxops = 0;
+  long work = 0;
+  for (int k = 0; k < n; ++k) {
+    const bca_batch_item& it = items[k];
+    const Mesh& m = ms[k];
+    BatchItem<T>& d = hi[k];
+    d.m = m;
+    d.ph = to_phys(&it.phys);
+    d.g = to_metric(it.g);
+    d.w = it.w; d.wd = it.wd; d.res = it.residu; d.resd = it.residud;
+    d.ws = p; p += 5 * m.ncell;
+    d.prim = p; p += NPRIM * m.ncell;
+    d.grad = p; p += NGRAD * m.ncell;
+    d.hn = p; p += 10 * m.nnode;
+    d.nops = it.nops;
+    plan->maxops = std::max(plan->maxops, it.nops);
+    for (int o = 0; o < it.nops; ++o) {
+      const bca_bc_op& op = it.ops[o];
+      d.ops[o] = BatchBc{op.kind, op.loc, Box{op.box[0], op.box[1], op.box[2], op.box[3]},
+                         Box{op.box2[0], op.box2[1], op.box2[2], op.box2[3]}, op.tr[0], op.tr[1], op.data, op.lm,
+                         op.gam};
+    }
+    plan->cells[k + 1] = plan->cells[k] + m.ncell;
+    plan->interior[k + 1] = plan->interior[k] + (long)m.im * m.jm;
+    plan->edges[k + 1] = plan->edges[k] + m.im + m.jm;
+    plan->faces[k + 1] = plan->faces[k] + (long)(m.im + 1) * (m.jm + 1);
+    work = std::max(work, plan->faces[k + 1]);
+  }
+  for (int o = 0; o < BATCH_OPS; ++o)
+    for (int k = 0; k < n; ++k)
+      plan->bc[o][k + 1] = plan->bc[o][k] + (o < items[k].nops ? bc_items(items[k].ops[o]) : 0);
+  BCA_HIP(hipMemcpyAsync(b->table, b->host, tbytes, hipMemcpyHostToDevice, b->stream));
+  if (!b->copied) BCA_HIP(hipEventCreateWithFlags(&b->copied, hipEventDisableTiming));
+  BCA_HIP(hipEventRecord(b->copied, b->stream));
+  b->pending = true;
+  // a resident grid: blocks per CU at this kernel's occupancy times the CUs,
+  // no more than the largest stage needs
+  int dev = 0, ncu = 0, per_cu = 0;
+  BCA_HIP(hipGetDevice(&dev));
+  BCA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  BCA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<T>, 256, 0));
+  if (per_cu < 1) return fail(-2, "batch kernel does not fit on a CU");
+  const unsigned grid = (unsigned)std::max(1L, std::min((long)per_cu * ncu, (work + 255) / 256));
+  const BatchItem<T>* ditems = reinterpret_cast<const BatchItem<T>*>(b->table);
+  const BatchPlan* dplan = reinterpret_cast<const BatchPlan*>(ditems + BATCH_MAX);
+  void* args[] = {(void*)&ditems, (void*)&dplan};
+  BCA_HIP(hipLaunchCooperativeKernel((const void*)k_batch<T>, dim3(grid), dim3(256), args, 0, b->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bca_batch_create(bca_batch** out, void* stream) {
+  if (!out) return fail(-1, "null argument");
+  *out = new bca_batch();
+  (*out)->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int bca_batch_set_stream(bca_batch* b, void* stream) {
+  if (!b) return fail(-1, "null batch");
+  b->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int bca_batch_destroy(bca_batch* b) {
+  if (!b) return 0;
+  if (b->pending) (void)hipEventSynchronize(b->copied);
+  if (b->copied) (void)hipEventDestroy(b->copied);
+  if (b->scratch) (void)hipFree(b->scratch);
+  if (b->table) (void)hipFree(b->table);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+  return 0;
+}
+
+int bca_batch_residual_jvp(bca_batch* b, const bca_batch_item* items, int n) {
+  if (!b || !items) return fail(-1, "null argument");
+  if (n >= 1 && items[0].wd) return run_batch<Dual<1>>(b, items, n);
+  return run_batch<double>(b, items, n);
 }
 
 }  // extern "C"

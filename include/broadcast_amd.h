@@ -227,6 +227,31 @@ int bca_jacobian_colored_2zones(bca_ctx *ctx, const double *w, bca_metrics g, co
                                 const bca_bc_op *ops, int nops, const double *coefdiag, int isplit,
                                 double *jac, int *ia, int *ja, long nbentry, int write_index);
 
+/* ---- uneven-mesh batch ----------------------------------------------------- */
+/* The reference's per-mesh iteration work -- the BC calls in order
+ * (BROADCAST_npz.py:1018-1021, tangent :1079-1082) then flux_num_dnc5_2d
+ * (srcfv/rhs/flux_num_dnc5.F90:7-226) or flux_num_dnc5_2d_d
+ * (srcfv/tangent/flux_num_dnc5_d.f90:15) -- for up to 16 meshes of different
+ * sizes (gh = 3) in one persistent cooperative launch.  Per item: the ghost
+ * frames of w (and wd) are filled in place by the BC sequence, residu (and
+ * residud = dR/dw . wd) are written on the interior.  Items carry wd and
+ * residud all or none (residual only).  No reference counterpart as a batch:
+ * the reference calls the per-mesh routines once per mesh. */
+typedef struct bca_batch bca_batch;
+typedef struct {
+  int im, jm;
+  bca_metrics g;
+  bca_phys phys;
+  const bca_bc_op *ops; /* host array, applied in order, at most 8 */
+  int nops;
+  double *w, *wd;        /* device, ni*nj*5 each (wd may be NULL) */
+  double *residu, *residud;
+} bca_batch_item;
+int bca_batch_create(bca_batch **batch, void *hip_stream);
+int bca_batch_set_stream(bca_batch *batch, void *hip_stream);
+int bca_batch_destroy(bca_batch *batch);
+int bca_batch_residual_jvp(bca_batch *batch, const bca_batch_item *items, int n);
+
 #ifdef __cplusplus
 }
 #endif
