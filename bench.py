@@ -295,6 +295,7 @@ def main():
     if not args.no_extras:
         out["newton_bl256x128"] = newton_figure()
         out["jbeta_bl512x192"] = jbeta_figure()
+        out["uneven_batch_bl8"] = uneven_batch_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out), flush=True)
@@ -334,6 +335,54 @@ def newton_figure():
     return {"seconds_to_1e-10": wall, "converged": log.converged, "newton_steps": len(log.norms) - 1,
             "norms": log.norms, "seconds_per_iteration": log.seconds, "cfl": 1e3,
             "start": "Blasius-type state + 1e-3 noise (seed 0), plain Newton"}
+
+
+def uneven_batch_figure(reps=20):
+    """BCs + residual + JVP (one dense tangent) of the 8 boundary-layer meshes
+    of log-uniform size (im 128..1024, jm 64..256, seeds 2000..2007): the
+    one-launch batch against the same work as a per-mesh loop of the
+    single-mesh entry points.  HBM figure at the dense-JVP algorithmic bytes,
+    216 B per cell (wd in + residud out 80 B, shared state and metrics 96 B,
+    residu out 40 B)."""
+    import numpy as np
+    import torch
+    from broadcast_amd.batch import UnevenBatch
+    from broadcast_amd.cases import uneven_bl_cases
+    from broadcast_amd.flow import Flow, to_device
+
+    flows = [Flow(c) for c in uneven_bl_cases()]
+    rng = np.random.default_rng(2100)
+    ws = [to_device(f.case.w) for f in flows]
+    wds = [to_device(rng.standard_normal(f.shape_w)) for f in flows]
+    res = [f.new_state() for f in flows]
+    resd = [f.new_state() for f in flows]
+    b = UnevenBatch(flows)
+    cells = b.cells
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_batch = timed(lambda: b.run(ws, wds, res, resd))
+
+    def per_mesh():
+        for k, f in enumerate(flows):
+            f.jvp(ws[k], wds[k], res[k], resd[k])
+
+    ms_loop = timed(per_mesh)
+    gbs = 216 * cells / (ms_batch * 1e-3) / 1e9
+    print(f"bench: uneven batch {cells} cells {ms_batch:.3f} ms (per-mesh loop {ms_loop:.3f} ms)",
+          file=sys.stderr, flush=True)
+    return {"meshes": [[f.im, f.jm] for f in flows], "cells": cells, "ms_batch": ms_batch,
+            "mcells_per_s": cells / (ms_batch * 1e-3) / 1e6, "ms_per_mesh_loop": ms_loop,
+            "achieved_gbs_216B_per_cell": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
 def jbeta_figure():

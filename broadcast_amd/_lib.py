@@ -78,6 +78,7 @@ _SIGS = {
     "bca_batch_set_stream": [_dp, _dp],
     "bca_batch_destroy": [_dp],
     "bca_batch_residual_jvp": [_dp, C.POINTER(BatchItem), C.c_int],
+    "bca_bc_adjoint": [_dp, _dp, _dp, C.POINTER(BcOp), _dp, _dp],
 }
 
 # Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).

@@ -19,6 +19,7 @@
 #include "csr.h"
 #include "dz.h"
 #include "batch.h"
+#include "bcadj.h"
 
 using namespace bca;
 
@@ -85,6 +86,7 @@ struct bca_ctx {
   double* vjpj = nullptr;   // j-faces
   double* jbeta_comp = nullptr;  // J(beta) components on the pattern
   long jbeta_cap = 0;
+  Dual<5>* wadj = nullptr;  // seeded state of the BC adjoints
 };
 
 namespace {
@@ -338,6 +340,7 @@ int bca_ctx_destroy(bca_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->csr_mask) (void)hipFree(c->csr_mask);
   if (c->jbeta_comp) (void)hipFree(c->jbeta_comp);
+  if (c->wadj) (void)hipFree(c->wadj);
   delete c;
   return 0;
 }
@@ -1088,6 +1091,40 @@ int bca_batch_residual_jvp(bca_batch* b, const bca_batch_item* items, int n) {
   if (!b || !items) return fail(-1, "null argument");
   if (n >= 1 && items[0].wd) return run_batch<Dual<1>>(b, items, n);
   return run_batch<double>(b, items, n);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ BC adjoints
+extern "C" {
+
+int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op, const double* nx,
+                   const double* ny) {
+  if (!c || !w || !wb || !op) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  int rc = bc_check(m, *op, nx, ny);
+  if (rc) return rc;
+  const Box b{op->box[0], op->box[1], op->box[2], op->box[3]};
+  if (op->kind == BCA_BC_JOIN) {
+    const Box prd{op->box2[0], op->box2[1], op->box2[2], op->box2[3]};
+    const bool overlap = b.imin <= prd.imax && prd.imin <= b.imax && b.jmin <= prd.jmax && prd.jmin <= b.jmax;
+    if (overlap) return fail(-1, "join adjoint needs disjoint receiving and donor boxes");
+    k_join_adjoint<<<grid1(bc_items(*op), 256), 256, 0, c->stream>>>(m, wb, b, prd, 5);
+    BCA_LAUNCHED();
+    return 0;
+  }
+  if ((rc = alloc_planes(c, &c->wadj, 5 * m.ncell))) return rc;
+  const int lmax = (int)bc_items(*op);
+  for (int d = 0; d < ADJ_REACH; ++d) {
+    k_adj_seed<<<grid1(m.ncell, 256), 256, 0, c->stream>>>(m, w, c->wadj, op->loc, b, d);
+    BCA_LAUNCHED();
+    if ((rc = launch_bc<Dual<5>>(c, c->wadj, *op, nx, ny))) return rc;
+    k_adj_accumulate<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, c->wadj, wb, op->loc, b, d, lmax);
+    BCA_LAUNCHED();
+  }
+  k_adj_clear_ghosts<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, wb, op->loc, b, lmax);
+  BCA_LAUNCHED();
+  return 0;
 }
 
 }  // extern "C"

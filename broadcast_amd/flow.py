@@ -189,6 +189,25 @@ class Flow:
                    "bca_vjp_dnc5")
         return wb
 
+    def adjoint_bcs(self, w, wb):
+        """Adjoint of the BC sequence, in place on wb: the ops in reverse
+        order, each pushing its ghost adjoints to the cells it reads and
+        zeroing them (the reference's bc_*_b / jn_match_2d_b, cylinder.py:
+        1121-1154).  w is the state the sequence produced (each BC then reads
+        the inputs it read in the forward sequence: no BC here reads a cell
+        that a later one writes)."""
+        h = self.ctx.sync_stream()
+        for op in reversed(list(self.ops)):
+            _lib.check(self.L.bca_bc_adjoint(h, _ptr(w), _ptr(wb), C.byref(op), _ptr(self.nx), _ptr(self.ny)),
+                       "bca_bc_adjoint")
+        return wb
+
+    def vjp_bcs(self, w, resb, wb=None):
+        """(d (R o BC) / dw)^T resb: the flux adjoint, then the BC sequence's
+        adjoint; the transpose of jvp(w, wd) as a map of wd."""
+        wb = self.vjp(w, resb, wb)
+        return self.adjoint_bcs(w, wb)
+
     def norms(self, res):
         out = (C.c_double * 10)()
         h = self.ctx.sync_stream()

@@ -69,3 +69,29 @@ def join(wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d, gh3d, g
     _lib.check(_lib.lib().bca_bc_join(ctx.h, st[0].ptr, dev.int4(prr), dev.int4(prd), tr_c, int(em),
                                       ni_arr, nj_arr), "jn_match_2d")
     dev.finish(st)
+
+
+# ------------------------------------------------------------------ adjoints
+def _op(kind, loc=0, interf=None, interf2=None, data=None, lm=0, gam=0.0):
+    op = _lib.BcOp()
+    op.kind, op.loc, op.gam, op.lm = kind, loc, float(gam), int(lm)
+    if interf is not None:
+        op.box[:] = list(dev.int4(interf))
+    if interf2 is not None:
+        op.box2[:] = list(dev.int4(interf2))
+    op.tr[:] = [1, 2]
+    op.data = data
+    return op
+
+
+def adjoint(kind, w, wb, im, jm, gh, loc=None, interf=None, interf2=None, data=None, lm=0, gam=0.0,
+            nx=None, ny=None, what="bc adjoint"):
+    """wb updated in place: the BC's ghost adjoints pushed to the cells it
+    reads, then zeroed (w is read only)."""
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(w, wb, data, nx, ny, out=(1,))
+    dp = st[2].ptr
+    op = _op(kind, dev.loc_code(loc) if loc is not None else 0, interf, interf2,
+             None if dp is None else dp.value, lm, gam)
+    _lib.check(_lib.lib().bca_bc_adjoint(ctx.h, st[0].ptr, st[1].ptr, C.byref(op), st[3].ptr, st[4].ptr), what)
+    dev.finish(st)

@@ -252,6 +252,22 @@ int bca_batch_set_stream(bca_batch *batch, void *hip_stream);
 int bca_batch_destroy(bca_batch *batch);
 int bca_batch_residual_jvp(bca_batch *batch, const bca_batch_item *items, int n);
 
+/* ---- f_adj boundary conditions --------------------------------------------- */
+/* bc_supandsubinlet_2d_b  srcfv/adjoint/bc_supandsubinlet_b.f90:11
+ * bc_no_reflexion_2d_b    srcfv/adjoint/bc_no_reflexion_b.f90:16
+ * bc_extrapolate_o2_2d_b  srcfv/adjoint/bc_extrapolateo2_b.f90:18
+ * bc_wall_viscous_adia_2d_b srcfv/adjoint/bc_wall_viscous_b.f90:11
+ * jn_match_2d_b           srcfv/adjoint/jn_match_2d_b.f90:11 (in-block join)
+ * One operation `op` (kind as in the BC sequence): wb += (d ghosts / d w)^T
+ * wb(ghosts) at the cells the BC reads, then wb(ghosts) = 0; w is unchanged
+ * (the reference restores it).  For the join, receiving and donor boxes in
+ * one block: wb(prd) += wb(prr), wb(prr) = 0 -- the transpose of the copy
+ * (the reference routine takes two arrays and zeroes the donor adjoint
+ * first, which in one block would clear wb).  nx, ny: the metrics (inlet and
+ * no-reflexion read them). */
+int bca_bc_adjoint(bca_ctx *ctx, const double *w, double *wb, const bca_bc_op *op, const double *nx,
+                   const double *ny);
+
 #ifdef __cplusplus
 }
 #endif

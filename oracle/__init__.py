@@ -287,6 +287,25 @@ class Reference:
                                                 *self._r(gam), _p(_i2(interf)),
                                                 *self._i(gh, im, jm), C.c_size_t(3))
 
+    # TAPENADE adjoints of the BCs (srcfv/adjoint/bc_*_b.f90): wb in/out, w restored
+    def bc_supandsubinlet_2d_b(self, w, wb, loc, interf, field, nx, ny, gam, im, jm, gh=3):
+        self.lib.bc_supandsubinlet_2d_b_(_p(w), _p(wb), C.c_char_p(loc.encode()), _p(_i2(interf)), _p(field),
+                                         _p(nx), _p(ny), *self._r(gam), *self._i(im, jm, field.shape[0], gh),
+                                         C.c_size_t(3))
+
+    def bc_no_reflexion_2d_b(self, w, wb, wbd, loc, interf, nx, ny, gam, gh, im, jm):
+        self.lib.bc_no_reflexion_2d_b_(_p(w), _p(wb), _p(wbd), C.c_char_p(loc.encode()), _p(_i2(interf)),
+                                       _p(nx), _p(ny), *self._r(gam), *self._i(gh, im, jm, wbd.shape[0]),
+                                       C.c_size_t(3))
+
+    def bc_extrapolate_o2_2d_b(self, w, wb, loc, interf, im, jm, gh):
+        self.lib.bc_extrapolate_o2_2d_b_(_p(w), _p(wb), C.c_char_p(loc.encode()), _p(_i2(interf)),
+                                         *self._i(im, jm, gh, w.shape[2]), C.c_size_t(3))
+
+    def bc_wall_viscous_adia_2d_b(self, w, wb, loc, gam, interf, gh, im, jm):
+        self.lib.bc_wall_viscous_adia_2d_b_(_p(w), _p(wb), C.c_char_p(loc.encode()), *self._r(gam),
+                                            _p(_i2(interf)), *self._i(gh, im, jm), C.c_size_t(3))
+
     def jn_match_2d(self, wr, prr, gh1r, gh2r, gh3r, gh4r, imr, jmr, wd, prd, gh1d, gh2d,
                     gh3d, gh4d, imd, jmd, tr):
         em = wr.shape[2] if wr.ndim == 3 else 1

@@ -90,6 +90,89 @@ def apply_bcs(lib, case: Case, g, w, wd=None):
                                 jm, tr)
 
 
+def _line_cells(loc, interf):
+    """Cells (i, j) of the lines of a BC interface and its inward normal
+    (i0, j0) (srcfv/borders/init_2d.F:1-44)."""
+    b = np.asarray(interf).astype(int)
+    (imin, jmin), (imax, jmax) = b[0], b[1]
+    i0 = {"Ilo": 1, "Ihi": -1}.get(loc, 0)
+    j0 = {"Jlo": 1, "Jhi": -1}.get(loc, 0)
+    if i0:
+        return [(imin, j) for j in range(jmin, jmax + 1)], i0, j0
+    return [(i, jmin) for i in range(imin, imax + 1)], i0, j0
+
+
+def adjoint_line_bc(case: Case, apply_d, w, wb, loc, interf):
+    """Adjoint of one line BC (the reference's bc_*_b, srcfv/adjoint/bc_*_b.f90)
+    formed as the transpose of its tangent: a BC line reads the cells at
+    offsets 0..3 along its inward normal and writes its gh ghost layers, so
+    for each offset d and variable q a unit tangent on that cell of every
+    line (lines are disjoint) gives the column d ghosts / d w(d, q), and
+    wb(d, q) += sum over the ghosts of column * wb(ghost).  Then wb(ghosts) = 0.
+    apply_d(w, wd) applies the tangent BC in place.  Returns the new wb."""
+    gh = case.gh
+    cells, i0, j0 = _line_cells(loc, interf)
+    I = lambda i: i + gh - 1
+    out = np.array(wb, order="F", copy=True)
+    ghosts = [[(I(i - de * i0), I(j - de * j0)) for de in range(1, gh + 1)] for i, j in cells]
+    for d in range(4):
+        for q in range(5):
+            wc = np.array(w, order="F", copy=True)
+            wd = np.zeros_like(wc, order="F")
+            for i, j in cells:
+                wd[I(i + d * i0), I(j + d * j0), q] = 1.0
+            apply_d(wc, wd)
+            for (i, j), gs in zip(cells, ghosts):
+                acc = sum(float(np.dot(wd[a, b, :], wb[a, b, :])) for a, b in gs)
+                out[I(i + d * i0), I(j + d * j0), q] += acc
+    for gs in ghosts:
+        for a, b in gs:
+            out[a, b, :] = 0.0
+    return out
+
+
+def adjoint_join(case: Case, wb, prr, prd):
+    """In-block join adjoint: the copy wb(prr) <- wb(prd) transposed."""
+    gh = case.gh
+    r = np.asarray(prr).astype(int)
+    d = np.asarray(prd).astype(int)
+    out = np.array(wb, order="F", copy=True)
+    rs = (slice(r[0, 0] + gh - 1, r[1, 0] + gh), slice(r[0, 1] + gh - 1, r[1, 1] + gh))
+    ds = (slice(d[0, 0] + gh - 1, d[1, 0] + gh), slice(d[0, 1] + gh - 1, d[1, 1] + gh))
+    out[ds] += out[rs]
+    out[rs] = 0.0
+    return out
+
+
+def adjoint_bcs(lib, case: Case, g, w, wb):
+    """Adjoint of the BC sequence of apply_bcs, ops in reverse order, at the
+    state w the sequence produced."""
+    im, jm, gh = case.im, case.jm, case.gh
+    gam = case.phys.gam
+    it = case.interf
+    ops = []
+    if case.kind == "bl":
+        ops.append(("Ilo", lambda a, ad: lib.bc_supandsubinlet_2d(a, "Ilo", it["Ilo"], case.field, g["nx"],
+                                                                  g["ny"], gam, im, jm, gh, wd=ad)))
+        ops.append(("Jhi", lambda a, ad: lib.bc_no_reflexion_2d(a, case.wbd, "Jhi", it["Jhi"], g["nx"], g["ny"],
+                                                                gam, gh, im, jm, wd=ad)))
+        ops.append(("Ihi", lambda a, ad: lib.bc_extrapolate_o2_2d(a, "Ihi", it["Ihi"], im, jm, gh, wd=ad)))
+        ops.append(("Jlo", lambda a, ad: lib.bc_wall_viscous_adia_2d(a, "Jlo", gam, it["Jlo"], gh, im, jm, wd=ad)))
+        for loc, fn in reversed(ops):
+            wb = adjoint_line_bc(case, fn, w, wb, loc, it[loc])
+        return wb
+    ops.append(("Jhi", lambda a, ad: lib.bc_no_reflexion_2d(a, case.wbd, "Jhi", it["Jhi"], g["nx"], g["ny"],
+                                                            gam, gh, im, jm, wd=ad)))
+    ops.append(("Jlo", lambda a, ad: lib.bc_wall_viscous_adia_2d(a, "Jlo", gam, it["Jlo"], gh, im, jm, wd=ad)))
+    (prr1, prd2, _), (prr2, prd1, _) = join_boxes(im, jm, gh)
+    for _ in range(2):
+        for a, b in ((prr2, prd1), (prr1, prd2)):
+            wb = adjoint_join(case, wb, a, b)
+    for loc, fn in reversed(ops):
+        wb = adjoint_line_bc(case, fn, w, wb, loc, it[loc])
+    return wb
+
+
 def residual(lib, case: Case, g, w):
     res = np.zeros_like(w, order="F")
     lib.flux_num_dnc5_2d(res, w, g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"],

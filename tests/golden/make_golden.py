@@ -9,7 +9,8 @@ repository root:
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
 
 `--dz-only` adds (or refreshes) the spanwise operators dz, dz2 of the case
-fixtures from their stored inputs without regenerating anything else.
+fixtures from their stored inputs without regenerating anything else;
+`--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b).
 
 Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
 """
@@ -91,6 +92,36 @@ def dz_vectors(R, fx):
     return dict(dz=dz, dz2=dz2)
 
 
+def bcb_vectors(R, fx, seed=21):
+    """Each BC adjoint of the case's sequence (srcfv/adjoint/bc_*_b.f90) on
+    the stored state after BCs and one seeded full-array adjoint bcb_wb:
+    bcb_<loc> = wb after that routine alone."""
+    g = {k: np.asfortranarray(fx[f"geo_{k}"]) for k in ("nx", "ny")}
+    w = np.asfortranarray(fx["w_bc"])
+    gh, im, jm = int(fx["gh"]), int(fx["im"]), int(fx["jm"])
+    gam = float(fx["phys"][3])
+    it = {k: fx[f"interf_{k}"] for k in ("Ilo", "Ihi", "Jlo", "Jhi")}
+    rng = np.random.default_rng(seed)
+    wb0 = np.asfortranarray(rng.standard_normal(w.shape))
+    calls = {
+        "Jhi": lambda a, b: R.bc_no_reflexion_2d_b(a, b, np.asfortranarray(fx["wbd"]), "Jhi", it["Jhi"], g["nx"],
+                                                   g["ny"], gam, gh, im, jm),
+        "Jlo": lambda a, b: R.bc_wall_viscous_adia_2d_b(a, b, "Jlo", gam, it["Jlo"], gh, im, jm),
+    }
+    if str(fx["kind"]) == "bl":
+        calls["Ilo"] = lambda a, b: R.bc_supandsubinlet_2d_b(a, b, "Ilo", it["Ilo"], np.asfortranarray(fx["field"]),
+                                                             g["nx"], g["ny"], gam, im, jm, gh)
+        calls["Ihi"] = lambda a, b: R.bc_extrapolate_o2_2d_b(a, b, "Ihi", it["Ihi"], im, jm, gh)
+    out = {"bcb_wb": wb0}
+    for loc, fn in calls.items():
+        a = np.array(w, order="F", copy=True)
+        b = np.array(wb0, order="F", copy=True)
+        fn(a, b)
+        assert np.array_equal(a, w), "the reference restores w"
+        out[f"bcb_{loc}"] = b
+    return out
+
+
 def jac_fixture(R, case):
     g = D.geometry(R, case)
     w = np.array(case.w, order="F", copy=True)
@@ -112,11 +143,11 @@ CASES = ("bl24x16", "bl13x9", "cyl28x12")
 
 def main():
     R = oracle.Reference()
-    if "--dz-only" in sys.argv:
+    if "--dz-only" in sys.argv or "--bcb-only" in sys.argv:
         for name in CASES:
             path = os.path.join(OUT, f"{name}.npz")
             fx = dict(np.load(path, allow_pickle=False))
-            fx.update(dz_vectors(R, fx))
+            fx.update(dz_vectors(R, fx) if "--dz-only" in sys.argv else bcb_vectors(R, fx))
             np.savez_compressed(path, **fx)
             print(f"updated {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
         return
@@ -129,6 +160,7 @@ def main():
     }
     for name in CASES:
         fixtures[name].update(dz_vectors(R, fixtures[name]))
+        fixtures[name].update(bcb_vectors(R, fixtures[name]))
     for name, fx in fixtures.items():
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **fx)

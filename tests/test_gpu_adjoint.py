@@ -201,3 +201,86 @@ def test_autograd_residual(torch_mod, name):
         tangent = fwAD.unpack_dual(out).tangent
     ok, err = close(_host(tangent, f.shape_w)[3:-3, 3:-3], fx["resd"][0][3:-3, 3:-3], 1e-12)
     assert ok, err
+
+
+# ---------------------------------------------------------------- BC adjoints
+@pytest.mark.parametrize("name", CASES)
+def test_bc_adjoints_dropin_vs_reference(torch_mod, name):
+    """f_adj.bc_*_b on numpy F-order arrays (the C ABI bca_bc_adjoint) against
+    the reference's TAPENADE adjoint vectors, each BC alone; w unchanged."""
+    from broadcast_amd.srcfv import f_adj
+
+    fx = load(name)
+    case = case_from(fx)
+    im, jm, gh, gam, it = case.im, case.jm, case.gh, case.phys.gam, case.interf
+    nx, ny = np.asfortranarray(fx["geo_nx"]), np.asfortranarray(fx["geo_ny"])
+    w = np.array(fx["w_bc"], order="F")
+    calls = {
+        "Jhi": lambda b: f_adj.bc_no_reflexion_2d_b(w, b, case.wbd, "Jhi", it["Jhi"], nx, ny, gam, gh, im, jm),
+        "Jlo": lambda b: f_adj.bc_wall_viscous_adia_2d_b(w, b, "Jlo", gam, it["Jlo"], gh, im, jm),
+    }
+    if case.kind == "bl":
+        calls["Ilo"] = lambda b: f_adj.bc_supandsubinlet_2d_b(w, b, "Ilo", it["Ilo"], case.field, nx, ny, gam, im, jm)
+        calls["Ihi"] = lambda b: f_adj.bc_extrapolate_o2_2d_b(w, b, "Ihi", it["Ihi"], im, jm, gh)
+    for loc, fn in calls.items():
+        b = np.array(fx["bcb_wb"], order="F", copy=True)
+        fn(b)
+        ok, err = close(b, fx[f"bcb_{loc}"], 1e-13)
+        assert ok, (loc, err)
+        assert np.array_equal(w, fx["w_bc"])
+
+
+def test_join_adjoint_dropin(torch_mod):
+    from broadcast_amd.cases import join_boxes
+    from broadcast_amd.srcfv import f_adj
+
+    case = cylinder_case(20, 8, seed=41)
+    gh = case.gh
+    rng = np.random.default_rng(42)
+    wb = np.asfortranarray(rng.standard_normal((case.ni, case.nj, 5)))
+    w = np.asfortranarray(np.array(case.w))
+    (prr1, prd2, tr), _ = join_boxes(case.im, case.jm, gh)
+    ref = D.adjoint_join(case, wb, prr1, prd2)
+    f_adj.jn_match_2d_b(w, wb, prr1, gh, gh, gh, gh, case.im, case.jm, w, wb, prd2, gh, gh, gh, gh, case.im,
+                        case.jm, tr)
+    np.testing.assert_array_equal(wb, ref)
+
+
+@pytest.mark.parametrize("mk", [lambda: bl_case(16, 10, seed=31), lambda: cylinder_case(22, 9, seed=32)])
+def test_bc_sequence_adjoint_matches_oracle(torch_mod, O, mk):
+    from broadcast_amd.flow import Flow, to_host
+
+    case = mk()
+    f = Flow(case)
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    v = np.asfortranarray(np.random.default_rng(43).standard_normal(w.shape))
+    ref = D.adjoint_bcs(O, case, g, w, v)
+    wb = _dev(v)
+    f.adjoint_bcs(_dev(w), wb)
+    ok, err = close(to_host(wb, f.shape_w), ref, 1e-13)
+    assert ok, err
+
+
+@pytest.mark.parametrize("mk", [lambda: bl_case(96, 48, seed=51), lambda: cylinder_case(70, 40, seed=52)])
+def test_vjp_with_bcs_dot_product(torch_mod, mk):
+    """(R o BC) adjoint against its tangent: <v, J u> = <J^T v, u> with J the
+    BC tangents + flux JVP (Flow.jvp) and J^T the flux VJP + BC adjoints in
+    reverse order (Flow.vjp_bcs), 1e-12 relative."""
+    from broadcast_amd.flow import Flow
+
+    case = mk()
+    f = Flow(case)
+    w = _dev(case.w)
+    f.apply_bcs(w)
+    rng = np.random.default_rng(53)
+    u = rng.standard_normal(f.shape_w) * np.abs(case.w)
+    v = np.zeros(f.shape_w)
+    v[3:-3, 3:-3, :] = rng.standard_normal((case.im, case.jm, 5))
+    ud = _dev(u)
+    _, Ju = f.jvp(w.clone(), ud)
+    JTv = f.vjp_bcs(w, _dev(v))
+    a = float((_dev(v) * Ju).sum())
+    b = float((JTv * _dev(u)).sum())
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)), (a, b)

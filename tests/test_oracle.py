@@ -386,3 +386,52 @@ def test_dz_scatter_matches_reference(O):
         outs.append((jac, ia, ja))
     for x, y in zip(*outs):
         np.testing.assert_array_equal(x, y)
+
+
+def _bc_calls(O, case, g):
+    im, jm, gh, gam, it = case.im, case.jm, case.gh, case.phys.gam, case.interf
+    calls = {
+        "Jhi": lambda a, ad: O.bc_no_reflexion_2d(a, case.wbd, "Jhi", it["Jhi"], g["nx"], g["ny"], gam, gh, im, jm,
+                                                  wd=ad),
+        "Jlo": lambda a, ad: O.bc_wall_viscous_adia_2d(a, "Jlo", gam, it["Jlo"], gh, im, jm, wd=ad),
+    }
+    if case.kind == "bl":
+        calls["Ilo"] = lambda a, ad: O.bc_supandsubinlet_2d(a, "Ilo", it["Ilo"], case.field, g["nx"], g["ny"], gam,
+                                                            im, jm, gh, wd=ad)
+        calls["Ihi"] = lambda a, ad: O.bc_extrapolate_o2_2d(a, "Ihi", it["Ihi"], im, jm, gh, wd=ad)
+    return calls
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_bc_adjoints_vs_reference(O, name):
+    """Transpose of the restated tangent BCs against the reference's TAPENADE
+    adjoints bc_*_b (golden vectors): each BC alone on one seeded wb."""
+    fx = load(name)
+    case = case_from(fx)
+    g = {k: fx[f"geo_{k}"] for k in ("nx", "ny")}
+    w = np.asfortranarray(fx["w_bc"])
+    for loc, fn in _bc_calls(O, case, g).items():
+        wb = D.adjoint_line_bc(case, fn, w, fx["bcb_wb"], loc, case.interf[loc])
+        ok, err = close(wb, fx[f"bcb_{loc}"], 1e-13)
+        assert ok, (loc, err)
+
+
+@pytest.mark.parametrize("mk", [lambda: bl_case(16, 10, seed=31), lambda: cylinder_case(22, 9, seed=32)])
+def test_bc_sequence_adjoint_dot_product(O, mk):
+    """<v, BCd(u)> = <BCb(v), u> for the whole BC sequence (joins included):
+    BCd replaces the ghost frame of u by the linearised BCs, BCb is the
+    sequence's adjoint in reverse order."""
+    case = mk()
+    g = D.geometry(O, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(O, case, g, w)
+    rng = np.random.default_rng(33)
+    u = np.asfortranarray(rng.standard_normal(w.shape))
+    v = np.asfortranarray(rng.standard_normal(w.shape))
+    wc, ud = np.array(w, order="F", copy=True), np.array(u, order="F", copy=True)
+    D.apply_bcs(O, case, g, wc, wd=ud)
+    vb = D.adjoint_bcs(O, case, g, w, v)
+    a, b = np.sum(v * ud), np.sum(vb * u)
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b))
+    # the ghost adjoints the sequence writes are consumed
+    assert np.abs(vb[:, :3]).max() == 0.0 and np.abs(vb[:, -3:]).max() == 0.0
