@@ -180,12 +180,32 @@ def main():
         f.jacobian_colored(w, coefdiag=cd, bufs=bufs, write_index=first)
         return f.csr(bufs[0])
 
+    def step_fused(first=False):
+        # the same matrix straight into the CSR, without the COO
+        f.apply_bcs(w)
+        f.residual(w, res)
+        return f.jacobian_csr(w, coefdiag=cd)
+
     print(f"bench: {case.name} cells={cells} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
           flush=True)
     nnz = int(step(first=True)[1].numel())
     for _ in range(max(args.warmup - 1, 0)):
         step()
+    step_fused(first=True)
     torch.cuda.synchronize()
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+    ec0, ec1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ec0.record()
+    for _ in range(max(args.steps, 1)):
+        step_fused()
+    ec1.record()
+    torch.cuda.synchronize()
+    fused_ms = ec0.elapsed_time(ec1) / max(args.steps, 1)
+    fused_kernels = {}
+    for name in ("k_jac_csr_count", "k_jac_csr_fill", "k_assemble_ghosts"):
+        tot, n = kernel_ms(f, name)
+        fused_kernels[name] = {"total_ms": tot, "launches": n, "avg_ms": tot / max(n, 1)}
+    f._csr_bufs = None  # release the fused path's CSR buffers
 
     # residual alone (same kernels as inside the step), for its own figure
     nres = 20
@@ -223,17 +243,19 @@ def main():
 
     # algorithmic bytes per launch of each Jacobian kernel
     nfi, nfj = (case.im + 1) * case.jm, case.im * (case.jm + 1)
+    blocks_read = 8 * 25 * 22 * (nfi + nfj)  # the 22 interior-stencil face blocks of 5 x 5 doubles
     alg = {
-        # Jacobian values written in the reference's COO layout (8 B each) and
-        # the 22 interior-stencil face blocks of 5 x 5 doubles read once per face
-        "k_assemble_colored": 8 * f.nbentry + 8 * 25 * 22 * (nfi + nfj),
         # state + metrics in (96 B/cell) and the 24 x 5 x 5 face-local blocks out
         "k_iface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfi - 2 * (case.im + 1)),
         "k_jface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfj - 3 * case.im),
-        # prune + CSR: the COO values read (count), read again with the CSR
-        # indices and values written (fill)
-        "k_csr_count": 8 * f.nbentry + 4 * 5 * cells,
-        "k_csr_fill": 8 * f.nbentry + 12 * nnz + 4 * 5 * cells,
+        # Jacobian values written in the reference's COO layout (8 B each) and
+        # the face blocks read once per face
+        "k_assemble_colored": 8 * f.nbentry + blocks_read,
+        # prune + CSR: the COO values read (count; the kept masks, 64 B per
+        # row, written), read again with the CSR indices and values written
+        # (fill)
+        "k_csr_count": 8 * f.nbentry + (64 + 4) * 5 * cells,
+        "k_csr_fill": 8 * f.nbentry + 12 * nnz + (64 + 4) * 5 * cells,
     }
     dom = max(alg, key=lambda k: kernels[k]["total_ms"])
     avg_dom = kernels[dom]["avg_ms"]
@@ -241,7 +263,7 @@ def main():
     avg64 = face64_ms / max(face64_n, 1)
     ach_res = RESIDUAL_B_PER_CELL * cells / (res_ms * 1e-3) / 1e9
     asm_ms = sum(v["total_ms"] for v in kernels.values()) / args.steps
-    ach_asm = (8 * f.nbentry + JVP_SHARED_B_PER_CELL * cells) / (asm_ms * 1e-3) / 1e9
+    ach_asm = (8 * nnz + JVP_SHARED_B_PER_CELL * cells) / (asm_ms * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -264,6 +286,7 @@ def main():
             "cells": cells,
             "seeds": nseeds,
             "jacobian_entries": f.nbentry,
+            "csr_nnz": nnz,
         },
         "roofline": {
             "kernel": dom,
@@ -280,8 +303,15 @@ def main():
         "jacobian_assembly": {
             "kernels": kernels,
             "ms_per_assembly_in_timed_kernels": asm_ms,
-            "achieved_gbs_8B_per_value_plus_96B_per_cell": ach_asm,
+            "achieved_gbs_8B_per_csr_value_plus_96B_per_cell": ach_asm,
             "frac": ach_asm / HBM_PEAK_GBS,
+        },
+        "fused_without_coo": {
+            "ms_per_step": fused_ms,
+            "kernels": fused_kernels,
+            "coo_bytes_not_allocated": 16 * f.nbentry,
+            "note": "same step straight into the CSR (bca_jacobian_colored_csr): same CSR bit for bit, "
+                    "no COO in memory; the slot values are formed twice (count, fill)",
         },
         "residual": {
             "ms": res_ms,

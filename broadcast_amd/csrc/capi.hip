@@ -20,6 +20,7 @@
 #include "dz.h"
 #include "batch.h"
 #include "bcadj.h"
+#include "jaccsr.h"
 
 using namespace bca;
 
@@ -87,6 +88,8 @@ struct bca_ctx {
   double* jbeta_comp = nullptr;  // J(beta) components on the pattern
   long jbeta_cap = 0;
   Dual<5>* wadj = nullptr;  // seeded state of the BC adjoints
+  double* ghostG = nullptr; // ghost part of the fused CSR path, per near row and slot color
+  long ghostG_cap = 0;
 };
 
 namespace {
@@ -341,6 +344,7 @@ int bca_ctx_destroy(bca_ctx* c) {
   if (c->csr_mask) (void)hipFree(c->csr_mask);
   if (c->jbeta_comp) (void)hipFree(c->jbeta_comp);
   if (c->wadj) (void)hipFree(c->wadj);
+  if (c->ghostG) (void)hipFree(c->ghostG);
   delete c;
   return 0;
 }
@@ -598,20 +602,17 @@ int ghost_tangents(bca_ctx* c, const double* w, const bca_metrics& g, const bca_
   return 0;
 }
 
-int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
-                  const double* coefdiag, const Zones& z, double* jac, int* ia, int* ja, long nbentry,
-                  int write_index) {
-  if (!c || !w || !p || !jac || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+// Phase 1 of the colored Jacobian: primitives and gradients of the state,
+// face-local blocks, ghost tangents of all zones x 49 colors.
+int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+                    const Zones& z) {
+  if (!c || !w || !p) return fail(-1, "null argument");
   if (!g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null metrics");
   if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
   const Mesh& m = c->m;
-  const int step = 2 * m.gh + 1;
-  if (nbentry < (long)z.n * m.im * m.jm * step * step * 25)
-    return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
   int rc = 0;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
   const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
-  const int nslot = z.n * step * step;
   if ((rc = alloc_planes(c, &c->blki, (long)NOFF * 25 * nfi))) return rc;
   if ((rc = alloc_planes(c, &c->blkj, (long)NOFF * 25 * nfj))) return rc;
   const Metric gm = to_metric(g);
@@ -646,7 +647,22 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   aj.j0 = aj.j1 = 1;
   launch_jface_blocks_wall(grid2(m.im, 1, b), b, s, aj);
   BCA_LAUNCHED();
-  if ((rc = ghost_tangents(c, w, g, ops, nops, z))) return rc;
+  return ghost_tangents(c, w, g, ops, nops, z);
+}
+
+int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+                  const double* coefdiag, const Zones& z, double* jac, int* ia, int* ja, long nbentry,
+                  int write_index) {
+  if (!c || !jac || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  if (nbentry < (long)z.n * m.im * m.jm * step * step * 25)
+    return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
+  int rc = jacobian_phase1(c, w, g, p, ops, nops, z);
+  if (rc) return rc;
+  const int nslot = z.n * step * step;
+  hipStream_t s = c->stream;
+  const dim3 b(64, 4);
   // phase 2: rows
   {
     Timed t(c, "k_assemble_colored");
@@ -660,7 +676,7 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   BCA_LAUNCHED();
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
+    k_assemble_ghosts<false><<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
                                                                               c->ghostnz, jac);
   }
   BCA_LAUNCHED();
@@ -1123,6 +1139,94 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
     BCA_LAUNCHED();
   }
   k_adj_clear_ghosts<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, wb, op->loc, b, lmax);
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ fused colored Jacobian -> CSR
+extern "C" {
+
+int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops,
+                             int nops, const double* coefdiag, int nzones, int zparam, double mini, int* indptr,
+                             int* indices, double* data, long cap, long* nnz) {
+  if (!c || !indptr || !nnz || (cap > 0 && (!indices || !data))) return fail(-1, "null argument");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  Zones z;
+  if (nzones == 1) {
+    z = Zones{1, {1, 0}, {m.im, 0}, zparam != 0};
+  } else if (nzones == 2) {
+    if (zparam < step || m.im - zparam < step) return fail(-1, "each zone must be at least 2gh+1 cells wide");
+    z = Zones{2, {1, zparam + 1}, {zparam, m.im}, 1};
+  } else {
+    return fail(-1, "nzones must be 1 or 2");
+  }
+  const long rows = 5L * m.im * m.jm;
+  if (rows * 245L * nzones > 2147483647L) return fail(-1, "matrix too large for 32-bit CSR indices");
+  int rc = jacobian_phase1(c, w, g, p, ops, nops, z);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const int nslot = z.n * step * step;
+  const long gsize = near_count(m) * nslot * 25;
+  if (c->ghostG_cap < gsize) {
+    if ((rc = realloc_planes(c, &c->ghostG, gsize))) return rc;
+    c->ghostG_cap = gsize;
+  }
+  {
+    Timed t(c, "k_assemble_ghosts");
+    k_assemble_ghosts<true><<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj,
+                                                                                     c->ghostT, c->ghostnz,
+                                                                                     c->ghostG);
+  }
+  BCA_LAUNCHED();
+  const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
+  if (nb > SCAN_B * SCAN_B) return fail(-1, "too many rows for the scan");
+  if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
+  int* sums = c->csr_aux;
+  int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
+  int* bad = total + 1;
+  k_fill_int<<<1, 64, 0, s>>>(bad, 1, 0);
+  BCA_LAUNCHED();
+  // 32-bit face-block offsets in the fused kernels
+  if ((long)(NOFF + 1) * 25 * std::max((long)(m.im + 1) * m.jm, (long)m.im * (m.jm + 1)) > 2147483647L)
+    return fail(-1, "mesh too large for the fused CSR path");
+  const dim3 gr((m.im + JC_TI - 1) / JC_TI, m.jm);
+  {
+    Timed t(c, "k_jac_csr_count");
+    const dim3 grc(gr.x, gr.y, 5);
+    if (z.n == 1)
+      k_jac_csr_count<1><<<grc, JCC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini,
+                                                     c->csr_mask, c->csr_cnt, bad);
+    else
+      k_jac_csr_count<2><<<grc, JCC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini,
+                                                     c->csr_mask, c->csr_cnt, bad);
+  }
+  BCA_LAUNCHED();
+  k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
+  BCA_LAUNCHED();
+  k_scan_sums<<<1, SCAN_B, 0, s>>>(sums, nb, total);
+  BCA_LAUNCHED();
+  k_scan_add<<<nb, SCAN_B, 0, s>>>(indptr, rows, sums, total);
+  BCA_LAUNCHED();
+  int host[2] = {0, 0};
+  BCA_HIP(hipMemcpyAsync(host, total, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  BCA_HIP(hipStreamSynchronize(s));
+  if (host[1]) return fail(-4, "a kept entry has a column outside the matrix");
+  *nnz = host[0];
+  if (host[0] > cap) return fail(-5, "capacity smaller than nnz (nnz returned; call again with larger arrays)");
+  {
+    Timed t(c, "k_jac_csr_fill");
+    if (z.n == 1)
+      k_jac_csr<1, true><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
+                                                   nullptr, indptr, indices, data, bad);
+    else
+      k_jac_csr<2, true><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
+                                                   nullptr, indptr, indices, data, bad);
+  }
   BCA_LAUNCHED();
   return 0;
 }

@@ -209,12 +209,18 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
       double v[7];
 #pragma unroll
       for (int D = 0; D < 7; ++D) {
+        // jac = -dR/dw with R(i,j) = hi(i) - hi(i+1) + hj(j) - hj(j+1); the
+        // loads are unguarded (offset 0 stands in for a missing block) so
+        // that they are all in flight at once, the selects keep the sum exact
+        const double x0 = bi[(pi0[D] >= 0 ? pi0[D] : 0) + q * nfi];
+        const double x1 = bi[(pi1[D] >= 0 ? pi1[D] : 0) + q * nfi];
+        const double x2 = bj[(pj0[D] >= 0 ? pj0[D] : 0) + q * nfj];
+        const double x3 = bj[(pj1[D] >= 0 ? pj1[D] : 0) + q * nfj];
         double acc = 0.0;
-        // jac = -dR/dw with R(i,j) = hi(i) - hi(i+1) + hj(j) - hj(j+1)
-        if (pi0[D] >= 0) acc -= bi[pi0[D] + q * nfi];
-        if (pi1[D] >= 0) acc += bi[pi1[D] + q * nfi];
-        if (pj0[D] >= 0) acc -= bj[pj0[D] + q * nfj];
-        if (pj1[D] >= 0) acc += bj[pj1[D] + q * nfj];
+        acc = pi0[D] >= 0 ? acc - x0 : acc;
+        acc = pi1[D] >= 0 ? acc + x1 : acc;
+        acc = pj0[D] >= 0 ? acc - x2 : acc;
+        acc = pj1[D] >= 0 ? acc + x3 : acc;
         v[D] = acc;
       }
 #pragma unroll
@@ -296,7 +302,23 @@ __device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, lon
   }
 }
 
-// Grid (x: near rows, y: zone * 49 + color).
+// index t of near_cell(m, t) = (i, j), or -1 when (i, j) is not a near row
+__device__ inline long near_index(const Mesh& m, int i, int j) {
+  const int W = NEAR;
+  if (near_whole(m)) return (long)(j - 1) * m.im + (i - 1);
+  if (j <= W) return (long)(j - 1) * m.im + (i - 1);
+  if (j > m.jm - W) return (long)W * m.im + (long)(j - (m.jm - W + 1)) * m.im + (i - 1);
+  long t = 2L * W * m.im + (long)(j - W - 1) * 2 * W;
+  if (i <= W) return t + (i - 1);
+  if (i > m.im - W) return t + (i - 1 - m.im + 2 * W);
+  return -1;
+}
+
+// Grid (x: near rows, y: zone * 49 + color).  COMPACT = false adds the
+// ghost part to the COO slots; COMPACT = true stores it as
+// G[(t * nslot + zone * 49 + color) * 25 + e * 5 + m] for the fused CSR path
+// (zeros for a slot outside the zone's window).
+template <bool COMPACT>
 __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const double* __restrict__ bi,
                                                          const double* __restrict__ bj,
                                                          const double* __restrict__ T,
@@ -310,7 +332,13 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   int i, j;
   near_cell(m, t, i, j);
   int vali, valj;
-  if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) return;
+  if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) {
+    if (COMPACT) {
+#pragma unroll
+      for (int q = 0; q < 25; ++q) jac[(t * gridDim.y + blockIdx.y) * 25 + q] = 0.0;
+    }
+    return;
+  }
   const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
   const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
   const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
@@ -327,6 +355,11 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   const long plane = (long)im * jm;
   const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step +
                     (long)zn * plane * 25 * step * step;
+  if (COMPACT) {
+#pragma unroll
+    for (int q = 0; q < 25; ++q) jac[(t * gridDim.y + blockIdx.y) * 25 + q] = acc[q];
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < 5; ++e)
 #pragma unroll

@@ -276,6 +276,37 @@ class Flow:
                                                _ptr(indices), _ptr(data)), "bca_csr_colored_fill")
         return indptr, indices[:nnz.value], data[:nnz.value]
 
+    def jacobian_csr(self, w, coefdiag=None, mini=2e-16):
+        """The colored Jacobian straight into the pruned CSR
+        (bca_jacobian_colored_csr): the (indptr, indices, data) that
+        csr(jacobian_colored(w, coefdiag)) gives, without forming the COO.
+        The value arrays are views of buffers kept for the next call (grown
+        when a larger nnz needs it)."""
+        t = self.t
+        n = 5 * self.im * self.jm
+        split = self.two_zone_split
+        nz, zp = (1, int(self.case.kind == "cyl")) if split is None else (2, split)
+        bufs = getattr(self, "_csr_bufs", None)
+        if bufs is None:
+            # ~145 entries per interior row (29-cell stencil x 5 variables)
+            cap = 160 * n
+            bufs = (t.empty(n + 1, dtype=t.int32, device="cuda"), t.empty(cap, dtype=t.int32, device="cuda"),
+                    t.empty(cap, dtype=t.float64, device="cuda"))
+        nnz = C.c_long()
+        for _ in range(2):
+            indptr, indices, data = bufs
+            h = self.ctx.sync_stream()
+            rc = self.L.bca_jacobian_colored_csr(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops,
+                                                 _ptr(coefdiag), nz, zp, float(mini), _ptr(indptr), _ptr(indices),
+                                                 _ptr(data), indices.numel(), C.byref(nnz))
+            if rc != -5:
+                break
+            bufs = (indptr, t.empty(nnz.value, dtype=t.int32, device="cuda"),
+                    t.empty(nnz.value, dtype=t.float64, device="cuda"))
+        _lib.check(rc, "bca_jacobian_colored_csr")
+        self._csr_bufs = bufs
+        return bufs[0], bufs[1][:nnz.value], bufs[2][:nnz.value]
+
     # ------------------------------------------------------------ 3D operators
     def dz_colored(self, w, bufs=None, write_index=True):
         """Dz and Dzz of the seed loop of BROADCAST_npz.py:1230-1245 in their

@@ -268,6 +268,19 @@ int bca_batch_residual_jvp(bca_batch *batch, const bca_batch_item *items, int n)
 int bca_bc_adjoint(bca_ctx *ctx, const double *w, double *wb, const bca_bc_op *op, const double *nx,
                    const double *ny);
 
+/* ---- fused colored Jacobian -> pruned CSR ---------------------------------- */
+/* The seed loop of BROADCAST_npz.py:1068-1127 (nzones = 1, zparam = withjn)
+ * or of cylinder.py:895-935 (nzones = 2, zparam = isplit as for
+ * bca_jacobian_colored_2zones) followed by remove_zero_jac + csr_matrix
+ * (BROADCAST_npz.py:129-135,1136-1150; misc/PETSc_func.py:85), without the
+ * COO: indptr (5*im*jm + 1), and when nnz <= cap, indices and data (nnz each),
+ * bit for bit what bca_jacobian_colored[_2zones] + bca_csr_colored_count/_fill
+ * give.  Returns -5 with *nnz set when cap is too small (call again with
+ * arrays of *nnz entries).  Synchronizes the stream once (nnz). */
+int bca_jacobian_colored_csr(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
+                             const bca_bc_op *ops, int nops, const double *coefdiag, int nzones, int zparam,
+                             double mini, int *indptr, int *indices, double *data, long cap, long *nnz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -281,6 +281,42 @@ def test_batched_jacobian_coo_matches_oracle_loop(torch_mod, O, mk):
     assert np.all(diff <= 1e-12 * np.maximum(rowmax, 1e-300)), float(np.max(diff / np.maximum(rowmax, 1e-300)))
 
 
+@pytest.mark.parametrize("mk", [lambda: bl_case(29, 18, seed=31),
+                                lambda: cylinder_case(28, 14, seed=32),
+                                lambda: cylinder_case(30, 12, seed=33),
+                                lambda: bl_case(120, 61, seed=34),
+                                lambda: cylinder_case(101, 40, seed=35)])
+def test_fused_jacobian_csr_equals_coo_path(torch_mod, mk):
+    """bca_jacobian_colored_csr (no COO) against the COO path (colored
+    assembly, then the device prune + CSR, itself bit-exact against scipy on
+    the oracle-pinned COO): indptr, indices and values bit for bit, including
+    the retry when the value arrays are too small."""
+    from broadcast_amd.flow import to_device
+
+    case = mk()
+    print(f"case {case.kind} {case.im}x{case.jm}", flush=True)
+    f = _flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    cd = to_device(np.random.default_rng(36).uniform(0.5, 1.5, (case.im, case.jm)))
+    jac, _, _ = f.jacobian_colored(w, coefdiag=cd)
+    ref = [a.cpu().numpy() for a in f.csr(jac)]
+    t = torch_mod
+    n = 5 * case.im * case.jm
+    f._csr_bufs = (t.empty(n + 1, dtype=t.int32, device="cuda"), t.empty(8, dtype=t.int32, device="cuda"),
+                   t.empty(8, dtype=t.float64, device="cuda"))
+    for _ in range(2):
+        got = [a.cpu().numpy() for a in f.jacobian_csr(w, coefdiag=cd)]
+        for a, b, what in zip(got, ref, ("indptr", "indices", "data")):
+            np.testing.assert_array_equal(a, b, err_msg=what)
+    # without the relaxation term
+    jac, _, _ = f.jacobian_colored(w)
+    ref = [a.cpu().numpy() for a in f.csr(jac)]
+    got = [a.cpu().numpy() for a in f.jacobian_csr(w)]
+    for a, b, what in zip(got, ref, ("indptr", "indices", "data")):
+        np.testing.assert_array_equal(a, b, err_msg=what)
+
+
 def test_two_zone_seed_and_scatter_match_oracle(torch_mod, O):
     """testvector_partial + computejacobianfromjv_relaxed_withjnandcheck
     through f_misc, bit-exact, on slots prefilled with a mix of zeros and
