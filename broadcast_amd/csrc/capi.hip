@@ -21,6 +21,7 @@
 #include "batch.h"
 #include "bcadj.h"
 #include "jaccsr.h"
+#include "gemm.h"
 
 using namespace bca;
 
@@ -1227,6 +1228,29 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
       k_jac_csr<2, true><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
                                                    nullptr, indptr, indices, data, bad);
   }
+  BCA_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ BF16 GEMM (learned guess)
+extern "C" {
+
+int bca_gemm_bf16(void* stream, int M, int N, int K, const unsigned short* A, long sam, long sak,
+                  const unsigned short* B, long sbk, long sbn, float* C, long scm, long scn, int accumulate,
+                  int ksplit) {
+  if (!A || !B || !C) return fail(-1, "null argument");
+  if (M < 0 || N < 0 || K < 0 || ksplit < 1 || ksplit > 4096) return fail(-1, "bad GEMM shape");
+  if (M == 0 || N == 0) return 0;
+  // 32-bit element offsets inside the kernel
+  const long span_a = (long)(M - 1) * std::labs(sam) + (long)(std::max(K, 1) - 1) * std::labs(sak);
+  const long span_b = (long)(std::max(K, 1) - 1) * std::labs(sbk) + (long)(N - 1) * std::labs(sbn);
+  const long span_c = (long)(M - 1) * std::labs(scm) + (long)(N - 1) * std::labs(scn);
+  if (std::max(span_a, std::max(span_b, span_c)) > 2147483647L) return fail(-1, "GEMM operand too large");
+  const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
+  const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ksplit);
+  k_gemm_bf16<<<grid, 256, 0, (hipStream_t)stream>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, accumulate);
   BCA_LAUNCHED();
   return 0;
 }

@@ -1,0 +1,396 @@
+"""Learned Newton initial guess for the cylinder baseflow.
+
+A small network maps the flow parameters (Re_D, M) to the FP64 conservative
+field on the O-mesh; Newton (broadcast_amd.newton, the reference's
+fixed_point iteration, cylinder.py:827-1045) then starts from that field
+instead of the uniform free stream, and falls back to the free-stream start
+when the guess does not converge.  No reference counterpart: the reference
+always starts from its initial state.
+
+Network (BF16 weights and activations on the matrix cores, FP32 accumulation,
+FP32 master weights and Adam):
+    x = (Re, M) scaled to [-1, 1]
+    h1 = silu(x W1^T + b1), h2 = silu(h1 W2^T + b2)      (width H)
+    c = h2 W3^T + b3                                      (K coefficients)
+    y = c P + mu                                          (normalised field)
+P (K x n, n = 5 im jm) is a trainable linear decoder initialised with the
+leading POD modes of the training fields, so most of the parameters sit in it
+(K n ~ 2M at 384 x 256 with K = 4, 250k at 96 x 64 with K = 8).  Every product,
+forward and backward (X W^T, dY^T X, dY W), is bca_gemm_bf16, the project's
+MFMA kernel; the elementwise parts (bias, silu, Adam) are torch glue.
+
+Training data: baseflows the GPU Newton solver computes at Sobol-sampled
+(Re, M) in [40, 180] x [0.1, 0.4], from the free stream.  The network learns
+the baseflow's departure from the free stream of the same (Re, M), which is
+known exactly (the energy alone varies as 1/M^2 over the range, the
+departure is small and smooth): guess = free stream + prediction.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import device as dev
+
+RE_RANGE = (40.0, 180.0)
+MACH_RANGE = (0.1, 0.4)
+
+
+# ------------------------------------------------------------------ sampling
+def sobol_params(n, seed=4, skip=0):
+    """n (Re, M) pairs of a scrambled 2-D Sobol sequence over RE_RANGE x
+    MACH_RANGE (points skip .. skip + n - 1 of the sequence)."""
+    from scipy.stats import qmc
+
+    s = qmc.Sobol(d=2, scramble=True, seed=seed)
+    total = 1 << max(1, int(math.ceil(math.log2(skip + n))))
+    u = s.random(total)[skip:skip + n]
+    lo = np.array([RE_RANGE[0], MACH_RANGE[0]])
+    hi = np.array([RE_RANGE[1], MACH_RANGE[1]])
+    return lo + u * (hi - lo)
+
+
+def scale_params(p):
+    """(Re, M) -> [-1, 1]^2."""
+    p = np.asarray(p, dtype=np.float64)
+    lo = np.array([RE_RANGE[0], MACH_RANGE[0]])
+    hi = np.array([RE_RANGE[1], MACH_RANGE[1]])
+    return 2.0 * (p - lo) / (hi - lo) - 1.0
+
+
+# ------------------------------------------------------------------ GEMM
+def mm(a, b, out=None, accumulate=False, ksplit=1):
+    """out (fp32, M x N) = a (bf16, M x K) @ b (bf16, K x N) on the matrix
+    cores (bca_gemm_bf16); a and b may be any strided views."""
+    t = dev.torch()
+    if a.dtype != t.bfloat16 or b.dtype != t.bfloat16:
+        raise TypeError("mm takes bfloat16 operands")
+    if a.dim() != 2 or b.dim() != 2 or a.shape[1] != b.shape[0]:
+        raise ValueError(f"shapes {tuple(a.shape)} @ {tuple(b.shape)}")
+    M, K = a.shape
+    N = b.shape[1]
+    if out is None:
+        out = (t.zeros if (accumulate or ksplit > 1) else t.empty)((M, N), dtype=t.float32, device=a.device)
+    L = _lib.lib()
+    _lib.check(L.bca_gemm_bf16(dev.current_stream_ptr(), M, N, K, C.c_void_p(a.data_ptr()), a.stride(0),
+                               a.stride(1), C.c_void_p(b.data_ptr()), b.stride(0), b.stride(1),
+                               C.c_void_p(out.data_ptr()), out.stride(0), out.stride(1), int(accumulate),
+                               int(ksplit)), "bca_gemm_bf16")
+    return out
+
+
+def _ksplit(K):
+    return max(1, min(1024, K // 2048))
+
+
+# ------------------------------------------------------------------ network
+@dataclass
+class GuessNet:
+    n: int                    # field length 5 im jm
+    width: int = 64
+    modes: int = 8
+    params: dict = field(default_factory=dict)    # fp32 master weights
+    scale: object = None      # per-entry field scale (n,), fp32
+    mean: object = None       # mean normalised field (n,), fp32
+
+    @staticmethod
+    def create(fields, width=64, modes=8, seed=0):
+        """Network for training fields (N x n, FP64 device tensor): field
+        normalisation, decoder initialised with the leading POD modes."""
+        t = dev.torch()
+        N, n = fields.shape
+        net = GuessNet(n=n, width=width, modes=modes)
+        f = fields.view(N, 5, -1)
+        s = f.abs().amax(dim=(0, 2)).clamp_min(1e-12)                 # per variable
+        net.scale = s.repeat_interleave(f.shape[2]).to(t.float32)
+        y = (fields / net.scale.double())
+        net.mean = y.mean(dim=0).to(t.float32)
+        yc = (y - net.mean.double())
+        _, _, vh = t.linalg.svd(yc, full_matrices=False)
+        k = min(modes, vh.shape[0])
+        P = t.zeros((modes, n), dtype=t.float32, device=fields.device)
+        P[:k] = vh[:k].to(t.float32)
+        g = t.Generator(device="cpu").manual_seed(seed)
+        rnd = lambda *shape: (t.randn(*shape, generator=g) * math.sqrt(1.0 / shape[-1])).to(fields.device)
+        net.params = {
+            "W1": rnd(width, 2), "b1": t.zeros(width, device=fields.device),
+            "W2": rnd(width, width), "b2": t.zeros(width, device=fields.device),
+            "W3": rnd(modes, width) * 0.1, "b3": t.zeros(modes, device=fields.device),
+            "P": P,
+        }
+        return net
+
+    def parameter_count(self):
+        return sum(int(v.numel()) for v in self.params.values())
+
+    # ---- forward / backward through the MFMA kernel
+    def forward(self, x):
+        """x: (B, 2) fp32 scaled parameters -> (normalised field (B, n) fp32,
+        cache for backward)."""
+        t = dev.torch()
+        p = self.params
+        bf = lambda a: a.to(t.bfloat16)
+        xb = bf(x)
+        z1 = mm(xb, bf(p["W1"]).t()) + p["b1"]
+        h1 = z1 * t.sigmoid(z1)
+        z2 = mm(bf(h1), bf(p["W2"]).t()) + p["b2"]
+        h2 = z2 * t.sigmoid(z2)
+        c = mm(bf(h2), bf(p["W3"]).t()) + p["b3"]
+        y = mm(bf(c), bf(p["P"]), ksplit=1) + self.mean
+        return y, (xb, z1, h1, z2, h2, c)
+
+    def backward(self, dy, cache):
+        """Gradients of the parameters for dL/dy = dy (B x n, fp32)."""
+        t = dev.torch()
+        p = self.params
+        bf = lambda a: a.to(t.bfloat16)
+        xb, z1, h1, z2, h2, c = cache
+        dyb = bf(dy)
+        g = {}
+        g["P"] = mm(bf(c).t(), dyb)                                    # K x n
+        dc = mm(dyb, bf(p["P"]).t(), ksplit=_ksplit(self.n))             # B x K
+        g["b3"] = dc.sum(0)
+        g["W3"] = mm(bf(dc).t(), bf(h2))                                 # K x H
+        dh2 = mm(bf(dc), bf(p["W3"]))                                    # B x H
+        s2 = t.sigmoid(z2)
+        dz2 = dh2 * (s2 * (1.0 + z2 * (1.0 - s2)))
+        g["b2"] = dz2.sum(0)
+        g["W2"] = mm(bf(dz2).t(), bf(h1))
+        dh1 = mm(bf(dz2), bf(p["W2"]))
+        s1 = t.sigmoid(z1)
+        dz1 = dh1 * (s1 * (1.0 + z1 * (1.0 - s1)))
+        g["b1"] = dz1.sum(0)
+        g["W1"] = mm(bf(dz1).t(), xb)
+        return g
+
+    def loss_and_grads(self, x, target):
+        """Mean squared error over the normalised field and its gradients."""
+        y, cache = self.forward(x)
+        diff = y - target
+        B = x.shape[0]
+        loss = float((diff * diff).mean())
+        grads = self.backward(diff * (2.0 / (B * self.n)), cache)
+        return loss, grads
+
+    def predict(self, params_phys):
+        """FP64 fields (B, n) for physical (Re, M) pairs."""
+        t = dev.torch()
+        x = t.as_tensor(scale_params(params_phys), dtype=t.float32, device=self.mean.device)
+        y, _ = self.forward(x.reshape(-1, 2))
+        return y.double() * self.scale.double()
+
+
+def train(net, params_phys, fields, steps=2000, lr=3e-3, log=None):
+    """Adam on the FP32 master weights against the normalised training fields
+    (fields: N x n FP64 device tensor).  Returns the loss history."""
+    t = dev.torch()
+    x = t.as_tensor(scale_params(params_phys), dtype=t.float32, device=fields.device)
+    target = (fields / net.scale.double()).to(t.float32)
+    m = {k: t.zeros_like(v) for k, v in net.params.items()}
+    v = {k: t.zeros_like(v) for k, v in net.params.items()}
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    hist = []
+    for it in range(1, steps + 1):
+        loss, g = net.loss_and_grads(x, target)
+        hist.append(loss)
+        a = lr * (0.5 * (1.0 + math.cos(math.pi * it / steps)) * 0.9 + 0.1)   # cosine to lr/10
+        for k, p in net.params.items():
+            m[k].mul_(b1).add_(g[k], alpha=1 - b1)
+            v[k].mul_(b2).addcmul_(g[k], g[k], value=1 - b2)
+            mh = m[k] / (1 - b1 ** it)
+            vh = v[k] / (1 - b2 ** it)
+            p.sub_(a * mh / (vh.sqrt() + eps))
+        if log is not None and (it % max(1, steps // 10) == 0 or it == 1):
+            log(f"train step {it}: loss {loss:.3e}")
+    return hist
+
+
+# ------------------------------------------------------------------ baseflows and Newton
+def interior_field(flow, w):
+    """(n,) view-copy of the interior of a flat state in (variable, j, i)
+    order, the layout of the network's output."""
+    return w.view(5, flow.nj, flow.ni)[:, 3:-3, 3:-3].reshape(-1).clone()
+
+
+def set_interior(flow, w, field_):
+    w.view(5, flow.nj, flow.ni)[:, 3:-3, 3:-3] = field_.view(5, flow.jm, flow.im)
+
+
+@dataclass
+class Solve:
+    seconds: float
+    iterations: int
+    converged: bool
+    start: str
+    norms: list
+
+
+def solve(flow, w, tol=1e-10, ite=40, cfl=1e21):
+    """Newton to max_e ||R_e||_2 < tol from the state in w (in place)."""
+    from .newton import newton
+
+    t = dev.torch()
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    log = newton(flow, w, ite=ite, cfl=cfl, tol=tol)
+    t.cuda.synchronize()
+    return Solve(time.perf_counter() - t0, len(log.norms) - 1, log.converged, "", log.norms)
+
+
+def make_flow(im, jm, re, mach):
+    from .cases import cylinder_case
+    from .flow import Flow
+
+    return Flow(cylinder_case(im, jm, mach=float(mach), reynolds=float(re), noise=0.0))
+
+
+def freestream(flow):
+    from .flow import to_device
+
+    return to_device(flow.case.w)
+
+
+def baseflows(im, jm, params_phys, tol=1e-10, log=None):
+    """Newton baseflows from the free stream at each (Re, M): (departures
+    from the free stream, N x n FP64 device tensor, list of Solve, the (Re, M)
+    pairs that converged -- a pair that does not is left out and logged)."""
+    t = dev.torch()
+    out, stats, used = [], [], []
+    for k, (re, ma) in enumerate(params_phys):
+        f = make_flow(im, jm, re, ma)
+        w = freestream(f)
+        free = interior_field(f, w)
+        s = solve(f, w, tol=tol)
+        s.start = "free stream"
+        if log is not None:
+            log(f"baseflow {k}: Re {re:.1f} M {ma:.3f} {s.iterations} Newton steps {s.seconds:.2f} s"
+                f"{'' if s.converged else ' NOT CONVERGED, left out'}")
+        if not s.converged:
+            continue
+        out.append(interior_field(f, w) - free)
+        stats.append(s)
+        used.append((re, ma))
+    if not out:
+        raise RuntimeError("no baseflow converged")
+    return t.stack(out), stats, np.array(used)
+
+
+def guess_field(flow, net, re, mach):
+    """Free stream of (Re, M) + the network's departure, (n,) FP64."""
+    return interior_field(flow, freestream(flow)) + net.predict([[re, mach]])[0]
+
+
+def solve_from_guess(flow, guess, tol=1e-10, ite=40):
+    """Newton from the learned guess (interior set, ghosts by the BCs); the
+    free-stream start is the fallback when that does not converge.  The time
+    includes the fallback's."""
+    t = dev.torch()
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    w = freestream(flow)
+    set_interior(flow, w, guess)
+    s = solve(flow, w, tol=tol, ite=ite)
+    s.start = "learned"
+    if not s.converged or not all(np.isfinite(s.norms)):
+        w = freestream(flow)
+        s2 = solve(flow, w, tol=tol, ite=ite)
+        s = Solve(time.perf_counter() - t0, s.iterations + s2.iterations, s2.converged, "learned, fell back",
+                  s.norms + s2.norms)
+    else:
+        s.seconds = time.perf_counter() - t0
+    return s, w
+
+
+# ------------------------------------------------------------------ experiment
+def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, seed=4, log=print):
+    """Baseflows at n_train Sobol points, the network trained on them, then
+    for n_test held-out Sobol points Newton to ||R|| < 1e-10 from the free
+    stream and from the learned guess.  Returns a JSON-able summary."""
+    t = dev.torch()
+    params = sobol_params(n_train + n_test, seed=seed)
+    train_p, test_p = params[:n_train], params[n_train:]
+    t0 = time.perf_counter()
+    fields, stats, train_p = baseflows(im, jm, train_p, log=log)
+    gen_s = time.perf_counter() - t0
+    net = GuessNet.create(fields, width=width, modes=modes, seed=seed)
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = train(net, train_p, fields, steps=steps, log=log)
+    t.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    rows = []
+    for re, ma in test_p:
+        f = make_flow(im, jm, re, ma)
+        w0 = freestream(f)
+        free = interior_field(f, w0)
+        plain = solve(f, w0)
+        t.cuda.synchronize()
+        t1 = time.perf_counter()
+        guess = guess_field(f, net, re, ma)       # inference is part of the learned start's time
+        t.cuda.synchronize()
+        infer_s = time.perf_counter() - t1
+        learned, wl = solve_from_guess(f, guess)
+        sol = interior_field(f, wl)
+        rows.append({
+            "re": float(re), "mach": float(ma),
+            "plain": {"seconds": plain.seconds, "newton_steps": plain.iterations, "converged": plain.converged},
+            "learned": {"seconds": learned.seconds + infer_s, "newton_steps": learned.iterations,
+                        "converged": learned.converged, "start": learned.start, "inference_s": infer_s},
+            "distance_to_baseflow": {"free_stream": float((free - sol).norm() / sol.norm()),
+                                     "guess": float((guess - sol).norm() / sol.norm())},
+            "same_baseflow": float((interior_field(f, w0) - sol).norm() / sol.norm()),
+        })
+        log(f"held out Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps {plain.seconds:.2f} s, "
+            f"learned {learned.iterations} steps {learned.seconds + infer_s:.2f} s ({learned.start}), "
+            f"guess distance {rows[-1]['distance_to_baseflow']['guess']:.2e}")
+    ps = np.array([r["plain"]["seconds"] for r in rows])
+    ls = np.array([r["learned"]["seconds"] for r in rows])
+    pi = np.array([r["plain"]["newton_steps"] for r in rows])
+    li = np.array([r["learned"]["newton_steps"] for r in rows])
+    return {
+        "mesh": [im, jm], "train": int(len(train_p)), "train_requested": n_train, "held_out": n_test,
+        "sobol_seed": seed,
+        "network": {"width": width, "modes": modes, "parameters": net.parameter_count(),
+                    "dtype": "bf16 operands on MFMA, fp32 accumulation, fp32 master weights"},
+        "baseflow_generation_s": gen_s, "training_s": train_s, "training_steps": steps,
+        "loss_first_last": [hist[0], hist[-1]],
+        "training_newton_steps_mean": float(np.mean([s.iterations for s in stats])),
+        "held_out_results": rows,
+        "plain_seconds_total": float(ps.sum()), "learned_seconds_total": float(ls.sum()),
+        "speedup_wall_total": float(ps.sum() / ls.sum()),
+        "speedup_wall_median": float(np.median(ps / ls)),
+        "newton_steps_mean_plain": float(pi.mean()), "newton_steps_mean_learned": float(li.mean()),
+        "all_converged": bool(all(r["plain"]["converged"] and r["learned"]["converged"] for r in rows)),
+        "fallbacks": int(sum(r["learned"]["start"] != "learned" for r in rows)),
+    }
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="learned Newton initial guess on the cylinder O-mesh")
+    ap.add_argument("--im", type=int, default=192)
+    ap.add_argument("--jm", type=int, default=128)
+    ap.add_argument("--train", type=int, default=96)
+    ap.add_argument("--test", type=int, default=16)
+    ap.add_argument("--width", type=int, default=64)
+    ap.add_argument("--modes", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    res = experiment(a.im, a.jm, a.train, a.test, a.width, a.modes, a.steps, log=lambda s: print(s, flush=True))
+    txt = json.dumps(res, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(txt + "\n")
+    print(json.dumps({k: v for k, v in res.items() if k != "held_out_results"}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

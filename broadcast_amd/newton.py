@@ -5,7 +5,8 @@ Per iteration, as the reference's `fixed_point` mode:
 * boundary conditions, residual and its norms (compute_norml2inf);
 * relaxation r = max over the first three variables of the norm ratios to
   the first iteration (:1062), coefdiag = r / dt * vol with
-  dt = cfl (yc(1,2) - yc(1,1)) / (1/M + 1) (:1011-1012, :1063-1066);
+  dt = cfl (yc(1,2) - yc(1,1)) / (1/M + 1) (:1011-1012, :1063-1066), on the
+  O-mesh dt = cfl |xc(1,2) - xc(1,1)| / (1/M + 1) (cylinder.py:842-894);
 * the colored Jacobian -dR/dw + coefdiag, pruned, in CSR (:1068-1150);
 * the linear solve J dw = R (MUMPS LU in the reference, :1152-1157; here the
   device block-banded LU below);
@@ -149,8 +150,14 @@ def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth
     log = NewtonLog() if log is None else log
     case = flow.case
     mach = case.phys.mach
-    yc = flow.yc.view(flow.nj, flow.ni)
-    dy = float(yc[4, 3] - yc[3, 3])
+    if case.kind == "cyl":
+        # cylinder.py:842: |xc(1,2) - xc(1,1)|, the radial spacing at i = 1
+        xc = flow.xc.view(flow.nj, flow.ni)
+        dy = abs(float(xc[4, 3] - xc[3, 3]))
+    else:
+        # BROADCAST_npz.py:1011: yc(1,2) - yc(1,1), the first cell height
+        yc = flow.yc.view(flow.nj, flow.ni)
+        dy = float(yc[4, 3] - yc[3, 3])
     vol_int = flow.vol.view(flow.nj, flow.ni)[3:-3, 3:-3]      # (jm, im)
     res = flow.new_state()
     bufs = flow.jacobian_buffers()

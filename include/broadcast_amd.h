@@ -281,6 +281,17 @@ int bca_jacobian_colored_csr(bca_ctx *ctx, const double *w, bca_metrics g, const
                              const bca_bc_op *ops, int nops, const double *coefdiag, int nzones, int zparam,
                              double mini, int *indptr, int *indices, double *data, long cap, long *nnz);
 
+/* ---- BF16 GEMM of the learned Newton guess ---------------------------------- */
+/* No reference counterpart (the reference has no learned part).  C (fp32,
+ * M x N) = A (bf16 bit patterns, M x K) * B (bf16, K x N), FP32 accumulation on
+ * the matrix cores; element (r, c) of each operand at r * s_r + c * s_c, so
+ * transposed operands need no copy.  accumulate = 1 adds to C.  ksplit > 1
+ * splits K over ksplit block rows that add their partial sums to C with
+ * float atomics (C must hold the initial value, e.g. zeros). */
+int bca_gemm_bf16(void *hip_stream, int M, int N, int K, const unsigned short *A, long sam, long sak,
+                  const unsigned short *B, long sbk, long sbn, float *C, long scm, long scn, int accumulate,
+                  int ksplit);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,111 @@
+"""Learned Newton guess: the BF16 MFMA GEMM (bca_gemm_bf16), the network's
+hand-written backward, and the guess against the free-stream start.
+
+* GEMM: against the CPU product in float64 of the same bf16-rounded operands,
+  within the FP32 summation bound |err| <= K 2^-24 sum_k |a_ik b_kj| (+ the
+  output's own rounding), for plain, transposed and split-K operands and sizes
+  that are not multiples of the 16 x 16 tile.
+* Backward: the gradients of the GPU network (every product through the MFMA
+  kernel, BF16 operands) against torch autograd on the CPU in float64 of the
+  same network, normwise per parameter within 3e-2 (BF16 operands carry
+  8 bits; a wrong transpose or term is O(1) off).
+* Guess: trained on Newton baseflows of a small O-mesh, on held-out (Re, M)
+  it is at most half as far from the converged baseflow as the free stream
+  (relative 2-norm of the interior field), and Newton from it reaches the
+  same baseflow (||R|| < 1e-10) in no more steps.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+
+    assert torch.cuda.is_available(), "no HIP device"
+    return torch
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb,ksplit", [(16, 16, 16, False, False, 1), (37, 53, 70, False, False, 1),
+                                                  (37, 53, 70, True, True, 1), (96, 8, 5000, False, True, 4),
+                                                  (5, 300, 3, True, False, 1), (64, 64, 1024, False, False, 8)])
+def test_gemm_matches_cpu(torch_mod, M, N, K, ta, tb, ksplit):
+    from broadcast_amd.learned import mm
+
+    t = torch_mod
+    g = t.Generator().manual_seed(M * 1000 + N + K)
+    a = t.randn(K, M, generator=g) if ta else t.randn(M, K, generator=g)
+    b = t.randn(N, K, generator=g) if tb else t.randn(K, N, generator=g)
+    ab, bb = a.to(t.bfloat16), b.to(t.bfloat16)
+    A = ab.cuda().t() if ta else ab.cuda()
+    B = bb.cuda().t() if tb else bb.cuda()
+    got = mm(A, B, ksplit=ksplit).cpu().double()
+    Ad = (ab.t() if ta else ab).double()
+    Bd = (bb.t() if tb else bb).double()
+    ref = Ad @ Bd
+    bound = K * 2.0 ** -24 * (Ad.abs() @ Bd.abs()) + 2.0 ** -24 * ref.abs() + 1e-30
+    assert bool(((got - ref).abs() <= bound).all()), float(((got - ref).abs() / bound).max())
+
+
+def test_network_backward_matches_cpu_autograd(torch_mod):
+    from broadcast_amd.learned import GuessNet
+
+    t = torch_mod
+    g = t.Generator().manual_seed(7)
+    N, n = 12, 5 * 20 * 12
+    fields = (t.randn(N, n, generator=g, dtype=t.float64) * 0.1 + 1.0).cuda()
+    net = GuessNet.create(fields, width=32, modes=6, seed=3)
+    for k in net.params:
+        net.params[k] = net.params[k] + 0.05 * t.randn(net.params[k].shape, generator=g).cuda()
+    x = (t.rand(N, 2, generator=g) * 2 - 1).cuda()
+    target = (fields / net.scale.double()).float()
+    loss, grads = net.loss_and_grads(x, target)
+    # the same network on the CPU in float64, gradients by autograd
+    P = {k: v.detach().cpu().double().requires_grad_(True) for k, v in net.params.items()}
+    xc, tc = x.cpu().double(), target.cpu().double()
+    silu = lambda z: z * t.sigmoid(z)
+    h1 = silu(xc @ P["W1"].t() + P["b1"])
+    h2 = silu(h1 @ P["W2"].t() + P["b2"])
+    c = h2 @ P["W3"].t() + P["b3"]
+    y = c @ P["P"] + net.mean.cpu().double()
+    ref_loss = ((y - tc) ** 2).mean()
+    ref_loss.backward()
+    assert abs(loss - float(ref_loss)) <= 3e-2 * float(ref_loss)
+    for k in P:
+        gg, rr = grads[k].cpu().double(), P[k].grad
+        err = float((gg - rr).norm() / max(float(rr.norm()), 1e-30))
+        assert err <= 3e-2, (k, err)
+
+
+def test_learned_guess_beats_free_stream(torch_mod):
+    from broadcast_amd import learned as L
+
+    im, jm = 40, 32
+    params = L.sobol_params(8, seed=11)
+    train_p, test_p = params[:6], params[6:]
+    print(f"train {train_p.tolist()} held out {test_p.tolist()}", flush=True)
+    fields, stats, train_p = L.baseflows(im, jm, train_p)
+    assert len(train_p) == 6
+    net = L.GuessNet.create(fields, width=32, modes=6, seed=0)
+    hist = L.train(net, train_p, fields, steps=800, lr=3e-3)
+    assert hist[-1] < 0.05 * hist[0]
+    for k, (re, ma) in enumerate(test_p):
+        f = L.make_flow(im, jm, re, ma)
+        w0 = L.freestream(f)
+        free = L.interior_field(f, w0)
+        guess = L.guess_field(f, net, re, ma)
+        plain = L.solve(f, w0)
+        learned, wl = L.solve_from_guess(f, guess)
+        sol = L.interior_field(f, wl)
+        e_guess = float((guess - sol).norm() / sol.norm())
+        e_free = float((free - sol).norm() / sol.norm())
+        print(f"Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps, learned {learned.iterations} steps "
+              f"({learned.start}); distance to the solution: free stream {e_free:.2e}, guess {e_guess:.2e}",
+              flush=True)
+        assert plain.converged and learned.converged
+        assert learned.start == "learned"
+        assert float((L.interior_field(f, w0) - sol).norm() / sol.norm()) < 1e-8   # the same baseflow
+        assert e_guess < 0.5 * e_free
+        assert learned.iterations <= plain.iterations
