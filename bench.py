@@ -326,6 +326,7 @@ def main():
         out["newton_bl256x128"] = newton_figure()
         out["jbeta_bl512x192"] = jbeta_figure()
         out["uneven_batch_bl8"] = uneven_batch_figure()
+        out["learned_newton"] = learned_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out), flush=True)
@@ -413,6 +414,23 @@ def uneven_batch_figure(reps=20):
     return {"meshes": [[f.im, f.jm] for f in flows], "cells": cells, "ms_batch": ms_batch,
             "mcells_per_s": cells / (ms_batch * 1e-3) / 1e6, "ms_per_mesh_loop": ms_loop,
             "achieved_gbs_216B_per_cell": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
+def learned_figure():
+    """Newton time to ||R|| < 1e-10, learned guess vs free-stream start, on a
+    short protocol that fits the bench: cylinder O-mesh 96x64, 16 training
+    baseflows and 4 held-out Sobol points (the measured full run, 192x128
+    with 96 + 16, is broadcast_amd.learned's own command)."""
+    from broadcast_amd import learned as L
+
+    res = L.experiment(96, 64, n_train=16, n_test=4, width=64, modes=8, steps=1500, log=lambda s: None)
+    print(f"bench: learned guess 96x64: plain {res['plain_seconds_total']:.2f} s, learned "
+          f"{res['learned_seconds_total']:.2f} s (x{res['speedup_wall_total']:.2f}), Newton steps "
+          f"{res['newton_steps_mean_plain']:.1f} -> {res['newton_steps_mean_learned']:.1f}", file=sys.stderr, flush=True)
+    keep = ("mesh", "train", "held_out", "network", "training_s", "plain_seconds_total", "learned_seconds_total",
+            "speedup_wall_total", "speedup_wall_median", "newton_steps_mean_plain", "newton_steps_mean_learned",
+            "all_converged", "fallbacks")
+    return {k: res[k] for k in keep}
 
 
 def jbeta_figure():
