@@ -659,6 +659,9 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   const int step = 2 * m.gh + 1;
   if (nbentry < (long)z.n * m.im * m.jm * step * step * 25)
     return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
+  // 32-bit face-block offsets in the assembly
+  if ((long)(NOFF + 1) * 25 * std::max((long)(m.im + 1) * m.jm, (long)m.im * (m.jm + 1)) > 2147483647L)
+    return fail(-1, "mesh too large for 32-bit face-block offsets");
   int rc = jacobian_phase1(c, w, g, p, ops, nops, z);
   if (rc) return rc;
   const int nslot = z.n * step * step;
@@ -667,12 +670,14 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   // phase 2: rows
   {
     Timed t(c, "k_assemble_colored");
-    dim3 gr = grid2(m.im, m.jm, b);
+    // 256 consecutive i per block: each COO slot plane gets 2 KB runs at a time
+    const dim3 ba(256, 1);
+    dim3 gr = grid2(m.im, m.jm, ba);
     gr.z = step;
     if (z.n == 1)
-      k_assemble_colored<1><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
+      k_assemble_colored<1><<<gr, ba, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
     else
-      k_assemble_colored<2><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
+      k_assemble_colored<2><<<gr, ba, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, jac, ia, ja, write_index);
   }
   BCA_LAUNCHED();
   {

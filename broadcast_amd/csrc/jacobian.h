@@ -151,7 +151,7 @@ __device__ inline int color_offset(int k, int j, int n, int gh) {
 // whole interior contribution of each seed.  Values -dR/dw (+ coefdiag on the
 // diagonal) go to the reference's COO slots (misc/ComputeJacobian.f90:503-570).
 template <int NZ>
-__global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const double* __restrict__ bi,
+__global__ void __launch_bounds__(256, NZ > 1 ? 4 : 3) k_assemble_colored(Mesh m, Zones z, const double* __restrict__ bi,
                                                           const double* __restrict__ bj,
                                                           const double* __restrict__ coefdiag,
                                                           double* __restrict__ jac, int* __restrict__ ia,
@@ -161,40 +161,36 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
   const int k = blockIdx.z;
   if (i > m.im || j > m.jm) return;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
-  const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
-  const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
-  const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
+  // 32-bit face-block offsets (the host checks that the block arrays fit)
+  const int nfi = (im + 1) * jm, nfj = im * (jm + 1);
+  const int fi0 = (i - 1) + (im + 1) * (j - 1), fj0 = (i - 1) + im * (j - 1);
   const int dj = color_offset(k, j, jm, gh);
   const bool jin = j + dj >= 1 && j + dj <= jm;
-  // wave-uniform plane offsets of the 4 faces for each di (-1: no block), and
-  // the zone of the column cell i + di (each cell belongs to one zone)
-  long pi0[7], pi1[7], pj0[7], pj1[7];
-  int zc[7];
+  // per column offset di = D - 3 (wave-uniform): the offset indices o + 1 of
+  // the four faces' blocks (0: no block), 5 bits each, and the zone of the
+  // column cell i + di in bit 20 (each cell belongs to one zone)
+  int code[7];
 #pragma unroll
   for (int D = 0; D < 7; ++D) {
     const int di = D - 3;
-    int o = ioff_index(di, dj);
-    pi0[D] = (jin && o >= 0 && ioff_live(m, o, i)) ? (long)o * 25 * nfi + fi0 : -1;
-    o = ioff_index(di - 1, dj);
-    pi1[D] = (jin && o >= 0 && ioff_live(m, o, i + 1)) ? (long)o * 25 * nfi + fi1 : -1;
-    o = joff_index(di, dj);
-    pj0[D] = (jin && o >= 0 && joff_live(m, o, j)) ? (long)o * 25 * nfj + fj0 : -1;
-    o = joff_index(di, dj - 1);
-    pj1[D] = (jin && o >= 0 && joff_live(m, o, j + 1)) ? (long)o * 25 * nfj + fj1 : -1;
-    if (i + di < 1 || i + di > im) pi0[D] = pi1[D] = pj0[D] = pj1[D] = -1;
-    zc[D] = (NZ > 1 && i + di > z.hi[0]) ? 1 : 0;
+    const bool in = jin && i + di >= 1 && i + di <= im;
+    int o0 = ioff_index(di, dj), o1 = ioff_index(di - 1, dj);
+    int o2 = joff_index(di, dj), o3 = joff_index(di, dj - 1);
+    o0 = (in && o0 >= 0 && ioff_live(m, o0, i)) ? o0 + 1 : 0;
+    o1 = (in && o1 >= 0 && ioff_live(m, o1, i + 1)) ? o1 + 1 : 0;
+    o2 = (in && o2 >= 0 && joff_live(m, o2, j)) ? o2 + 1 : 0;
+    o3 = (in && o3 >= 0 && joff_live(m, o3, j + 1)) ? o3 + 1 : 0;
+    code[D] = o0 | (o1 << 5) | (o2 << 10) | (o3 << 15) | ((NZ > 1 && i + di > z.hi[0]) ? 1 << 20 : 0);
   }
-  // seed columns of the 7 colors (l, k) of this row, per zone
+  // seed columns of the 7 colors (l, k) of this row, per zone (-1: dummy slot)
   int col[NZ][7];
-  bool ok[NZ][7];
   const int rot = (i - 1) % step;
 #pragma unroll
   for (int zn = 0; zn < NZ; ++zn)
 #pragma unroll
     for (int l = 0; l < 7; ++l) {
       int vali, valj;
-      ok[zn][l] = zone_column(m, z, zn, i, j, l, k, vali, valj);
-      col[zn][l] = ok[zn][l] ? valj * 5 + vali * jm * 5 : 0;
+      col[zn][l] = zone_column(m, z, zn, i, j, l, k, vali, valj) ? valj * 5 + vali * jm * 5 : -1;
     }
   const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
   const long plane = (long)im * jm;
@@ -210,17 +206,18 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
 #pragma unroll
       for (int D = 0; D < 7; ++D) {
         // jac = -dR/dw with R(i,j) = hi(i) - hi(i+1) + hj(j) - hj(j+1); the
-        // loads are unguarded (offset 0 stands in for a missing block) so
-        // that they are all in flight at once, the selects keep the sum exact
-        const double x0 = bi[(pi0[D] >= 0 ? pi0[D] : 0) + q * nfi];
-        const double x1 = bi[(pi1[D] >= 0 ? pi1[D] : 0) + q * nfi];
-        const double x2 = bj[(pj0[D] >= 0 ? pj0[D] : 0) + q * nfj];
-        const double x3 = bj[(pj1[D] >= 0 ? pj1[D] : 0) + q * nfj];
+        // loads are unguarded (a missing block reads offset 0) so that they
+        // are all in flight at once, the selects keep the sum exact
+        const int o0 = code[D] & 31, o1 = (code[D] >> 5) & 31, o2 = (code[D] >> 10) & 31, o3 = (code[D] >> 15) & 31;
+        const double x0 = bi[(o0 ? (o0 - 1) * 25 * nfi + fi0 : 0) + q * nfi];
+        const double x1 = bi[(o1 ? (o1 - 1) * 25 * nfi + fi0 + 1 : 0) + q * nfi];
+        const double x2 = bj[(o2 ? (o2 - 1) * 25 * nfj + fj0 : 0) + q * nfj];
+        const double x3 = bj[(o3 ? (o3 - 1) * 25 * nfj + fj0 + im : 0) + q * nfj];
         double acc = 0.0;
-        acc = pi0[D] >= 0 ? acc - x0 : acc;
-        acc = pi1[D] >= 0 ? acc + x1 : acc;
-        acc = pj0[D] >= 0 ? acc - x2 : acc;
-        acc = pj1[D] >= 0 ? acc + x3 : acc;
+        acc = o0 ? acc - x0 : acc;
+        acc = o1 ? acc + x1 : acc;
+        acc = o2 ? acc - x2 : acc;
+        acc = o3 ? acc + x3 : acc;
         v[D] = acc;
       }
 #pragma unroll
@@ -230,22 +227,23 @@ __global__ void __launch_bounds__(256) k_assemble_colored(Mesh m, Zones z, const
         r += r < -3 ? 7 : 0;
         r -= r > 3 ? 7 : 0;
         double x = v[0];
-        int xz = zc[0];
+        int xz = code[0] >> 20;
 #pragma unroll
         for (int D = 1; D < 7; ++D) {
           x = (r == D - 3) ? v[D] : x;
-          xz = (r == D - 3) ? zc[D] : xz;
+          xz = (r == D - 3) ? code[D] >> 20 : xz;
         }
 #pragma unroll
         for (int zn = 0; zn < NZ; ++zn) {
           const long cur = base + (long)e * plane + (long)l * plane * 5 * step +
                            (long)mm * plane * 5 * step * step + (long)zn * plane * 25 * step * step;
-          const int c = ok[zn][l] ? col[zn][l] + mm : 0;
-          double val = (ok[zn][l] && xz == zn) ? x : 0.0;
-          if (ok[zn][l] && coefdiag && row == c) val += cd;
+          const bool ok = col[zn][l] >= 0;
+          const int c = ok ? col[zn][l] + mm : 0;
+          double val = (ok && xz == zn) ? x : 0.0;
+          if (ok && coefdiag && row == c) val += cd;
           jac[cur] = val;
           if (write_index) {
-            ia[cur] = ok[zn][l] ? row : dummy;
+            ia[cur] = ok ? row : dummy;
             ja[cur] = c;
           }
         }
