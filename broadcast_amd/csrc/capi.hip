@@ -682,7 +682,7 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   BCA_LAUNCHED();
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<false><<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
+    k_assemble_ghosts<false><<<dim3(grid1(near_count(m), 256), nslot * 5), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
                                                                               c->ghostnz, jac);
   }
   BCA_LAUNCHED();
@@ -1182,7 +1182,7 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
   }
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<true><<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj,
+    k_assemble_ghosts<true><<<dim3(grid1(near_count(m), 256), nslot * 5), 256, 0, s>>>(m, z, c->blki, c->blkj,
                                                                                      c->ghostT, c->ghostnz,
                                                                                      c->ghostG);
   }

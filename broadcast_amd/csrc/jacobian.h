@@ -272,34 +272,6 @@ __device__ inline void near_cell(const Mesh& m, long t, int& i, int& j) {
   i = 1 + (s < W ? s : m.im - 2 * W + s);
 }
 
-template <bool JFAM>
-__device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, long f, int fi, int fj, double sg,
-                                  const double* Tc, const unsigned char* nzc, double acc[25]) {
-  for (int o = 0; o < NOFF; ++o) {
-    if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
-    const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
-    const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
-    if (interior(m, ci, cj)) continue;
-    if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
-    const long g = ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
-    if (!nzc[g]) continue;
-    const double* tg = Tc + g * 25;
-    const double* b = blk + (long)o * 25 * nf + f;
-    double bb[25];
-#pragma unroll
-    for (int q = 0; q < 25; ++q) bb[q] = b[q * nf];
-#pragma unroll
-    for (int e = 0; e < 5; ++e)
-#pragma unroll
-      for (int mm = 0; mm < 5; ++mm) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) s += bb[e * 5 + q] * tg[q * 5 + mm];
-        acc[e * 5 + mm] += sg * s;
-      }
-  }
-}
-
 // index t of near_cell(m, t) = (i, j), or -1 when (i, j) is not a near row
 __device__ inline long near_index(const Mesh& m, int i, int j) {
   const int W = NEAR;
@@ -312,8 +284,38 @@ __device__ inline long near_index(const Mesh& m, int i, int j) {
   return -1;
 }
 
-// Grid (x: near rows, y: zone * 49 + color).  COMPACT = false adds the
-// ghost part to the COO slots; COMPACT = true stores it as
+// Row e of the ghost part of one face: acc[m] += sg sum_q B(face, o)[e][q]
+// T(color, g)[q][m] over the face's stencil offsets o whose cell g is a ghost
+// with a nonzero tangent.
+template <bool JFAM>
+__device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, long f, int fi, int fj, double sg,
+                                  const double* Tc, const unsigned char* nzc, int e, double acc[5]) {
+  for (int o = 0; o < NOFF; ++o) {
+    if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
+    const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
+    const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
+    if (interior(m, ci, cj)) continue;
+    if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
+    const long g = ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
+    if (!nzc[g]) continue;
+    const double* tg = Tc + g * 25;
+    const double* b = blk + ((long)o * 25 + e * 5) * nf + f;
+    double bb[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) bb[q] = b[q * nf];
+#pragma unroll
+    for (int mm = 0; mm < 5; ++mm) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) s += bb[q] * tg[q * 5 + mm];
+      acc[mm] += sg * s;
+    }
+  }
+}
+
+// Grid (x: near rows, y: (zone * 49 + color) * 5 + e), one thread per near
+// row cell, slot color and variable row e.  COMPACT = false adds the ghost
+// part to the COO slots; COMPACT = true stores it as
 // G[(t * nslot + zone * 49 + color) * 25 + e * 5 + m] for the fused CSR path
 // (zeros for a slot outside the zone's window).
 template <bool COMPACT>
@@ -325,15 +327,17 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= near_count(m)) return;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
-  const int color = blockIdx.y % (step * step), zn = blockIdx.y / (step * step);
+  const int slot = blockIdx.y / 5, e = blockIdx.y % 5, nslot = gridDim.y / 5;
+  const int color = slot % (step * step), zn = slot / (step * step);
   const int l = color / step, k = color % step;
   int i, j;
   near_cell(m, t, i, j);
   int vali, valj;
+  double* gout = jac + (t * nslot + slot) * 25 + e * 5;
   if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) {
     if (COMPACT) {
 #pragma unroll
-      for (int q = 0; q < 25; ++q) jac[(t * gridDim.y + blockIdx.y) * 25 + q] = 0.0;
+      for (int mm = 0; mm < 5; ++mm) gout[mm] = 0.0;
     }
     return;
   }
@@ -341,27 +345,23 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
   const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
   const long ng = ghost_count(m);
-  const double* Tc = T + (long)blockIdx.y * ng * 25;
-  const unsigned char* nzc = nz + (long)blockIdx.y * ng;
-  double acc[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) acc[q] = 0.0;
-  ghost_face<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, nzc, acc);
-  ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, acc);
-  ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, acc);
-  ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, acc);
-  const long plane = (long)im * jm;
-  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step +
-                    (long)zn * plane * 25 * step * step;
+  const double* Tc = T + (long)slot * ng * 25;
+  const unsigned char* nzc = nz + (long)slot * ng;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  ghost_face<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, nzc, e, acc);
+  ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, e, acc);
+  ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, e, acc);
+  ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, e, acc);
   if (COMPACT) {
 #pragma unroll
-    for (int q = 0; q < 25; ++q) jac[(t * gridDim.y + blockIdx.y) * 25 + q] = acc[q];
+    for (int mm = 0; mm < 5; ++mm) gout[mm] = acc[mm];
     return;
   }
+  const long plane = (long)im * jm;
+  const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step +
+                    (long)zn * plane * 25 * step * step + (long)e * plane;
 #pragma unroll
-  for (int e = 0; e < 5; ++e)
-#pragma unroll
-    for (int mm = 0; mm < 5; ++mm) jac[base + (long)e * plane + (long)mm * plane * 5 * step * step] += acc[e * 5 + mm];
+  for (int mm = 0; mm < 5; ++mm) jac[base + (long)mm * plane * 5 * step * step] += acc[mm];
 }
 
 }  // namespace bca
