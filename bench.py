@@ -173,6 +173,10 @@ def main():
 
     nzones = 1 if f.two_zone_split is None else 2
     nseeds = 245 * nzones
+    # launch shapes of the residual faces, face blocks and assembly for this mesh
+    f.apply_bcs(w)
+    tuned = f.autotune(w, jac=bufs[0])
+    print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
 
     def step(first=False):
         f.apply_bcs(w)
@@ -303,6 +307,7 @@ def main():
             "seeds": nseeds,
             "jacobian_entries": f.nbentry,
             "csr_nnz": nnz,
+            "launch_shapes": tuned,
         },
         "roofline": {
             "kernel": dom,

@@ -81,6 +81,8 @@ _SIGS = {
     "bca_bc_adjoint": [_dp, _dp, _dp, C.POINTER(BcOp), _dp, _dp],
     "bca_jacobian_colored_csr": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp, C.c_int,
                                  C.c_int, C.c_double, _dp, _dp, _dp, C.c_long, C.POINTER(C.c_long)],
+    "bca_autotune": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, C.c_int, C.c_int, _dp, C.c_long,
+                     C.c_int, C.c_char_p, C.c_int],
     "bca_gemm_bf16": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp,
                       C.c_long, C.c_long, C.c_int, C.c_int],
 }

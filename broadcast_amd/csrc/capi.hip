@@ -91,6 +91,10 @@ struct bca_ctx {
   Dual<5>* wadj = nullptr;  // seeded state of the BC adjoints
   double* ghostG = nullptr; // ghost part of the fused CSR path, per near row and slot color
   long ghostG_cap = 0;
+  // launch shapes chosen by bca_autotune for this mesh (defaults until then)
+  dim3 tune_face{64, 4, 1};      // k_face<double>
+  dim3 tune_blocks{64, 4, 1};    // interior face-block kernels of phase 1
+  dim3 tune_assemble{256, 1, 1}; // k_assemble_colored
 };
 
 namespace {
@@ -197,7 +201,7 @@ int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g
   BCA_LAUNCHED();
   k_grad_ghost<T><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, wk.grad);
   BCA_LAUNCHED();
-  const dim3 bf(64, sizeof(T) > 8 ? 1 : 4);
+  const dim3 bf = sizeof(T) > 8 ? dim3(64, 1) : c->tune_face;
   {
     Timed t(c, "k_face" + sfx);
     k_face<T><<<grid2(m.im + 1, m.jm + 1, bf), bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
@@ -629,9 +633,10 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BCA_LAUNCHED();
   // phase 1: face-local blocks (independent of the zones)
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
+  const dim3 bt = c->tune_blocks;
   {
     Timed t(c, "k_iface_blocks");
-    launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
+    launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, bt), bt, s, ai);
   }
   ai.j0 = 1; ai.j1 = 2;
   launch_iface_blocks_o2(grid2(m.im + 1, 2, b), b, s, ai);
@@ -639,7 +644,7 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, nullptr, 4, m.jm + 1};
   {
     Timed t(c, "k_jface_blocks");
-    launch_jface_blocks_c(grid2(m.im, m.jm - 2, b), b, s, aj);
+    launch_jface_blocks_c(grid2(m.im, m.jm - 2, bt), bt, s, aj);
   }
   aj.j0 = aj.j1 = 3;
   launch_jface_blocks_r3(grid2(m.im, 1, b), b, s, aj);
@@ -670,8 +675,7 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   // phase 2: rows
   {
     Timed t(c, "k_assemble_colored");
-    // 256 consecutive i per block: each COO slot plane gets 2 KB runs at a time
-    const dim3 ba(256, 1);
+    const dim3 ba = c->tune_assemble;
     dim3 gr = grid2(m.im, m.jm, ba);
     gr.z = step;
     if (z.n == 1)
@@ -1257,6 +1261,118 @@ int bca_gemm_bf16(void* stream, int M, int N, int K, const unsigned short* A, lo
   const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)ksplit);
   k_gemm_bf16<<<grid, 256, 0, (hipStream_t)stream>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, accumulate);
   BCA_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+// median of reps HIP-event timings of launch() on the context stream
+template <class F>
+int time_launch(bca_ctx* c, int reps, F&& launch, float& ms) {
+  hipEvent_t a, b;
+  BCA_HIP(hipEventCreate(&a));
+  BCA_HIP(hipEventCreate(&b));
+  std::vector<float> t;
+  for (int r = 0; r < reps; ++r) {
+    BCA_HIP(hipEventRecord(a, c->stream));
+    launch();
+    BCA_HIP(hipEventRecord(b, c->stream));
+    BCA_HIP(hipEventSynchronize(b));
+    float x = 0.f;
+    BCA_HIP(hipEventElapsedTime(&x, a, b));
+    t.push_back(x);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  std::sort(t.begin(), t.end());
+  ms = t[t.size() / 2];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bca_autotune(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+                 int nzones, int zparam, double* jac, long nbentry, int reps, char* report, int report_len) {
+  if (!c || !w || !p || !jac) return fail(-1, "null argument");
+  if (reps < 1 || reps > 100) return fail(-1, "reps must be 1..100");
+  const Mesh& m = c->m;
+  const int step = 2 * m.gh + 1;
+  Zones z;
+  if (nzones == 1) {
+    z = Zones{1, {1, 0}, {m.im, 0}, zparam != 0};
+  } else if (nzones == 2) {
+    if (zparam < step || m.im - zparam < step) return fail(-1, "each zone must be at least 2gh+1 cells wide");
+    z = Zones{2, {1, zparam + 1}, {zparam, m.im}, 1};
+  } else {
+    return fail(-1, "nzones must be 1 or 2");
+  }
+  if (nbentry < (long)z.n * m.im * m.jm * step * step * 25)
+    return fail(-1, "nbentry smaller than nzones*im*jm*(2gh+1)^2*25");
+  if ((long)(NOFF + 1) * 25 * std::max((long)(m.im + 1) * m.jm, (long)m.im * (m.jm + 1)) > 2147483647L)
+    return fail(-1, "mesh too large for 32-bit face-block offsets");
+  // inputs of every tuned kernel: primitives, gradients, blocks, ghost tangents
+  int rc = jacobian_phase1(c, w, g, p, ops, nops, z);
+  if (rc) return rc;
+  const Metric gm = to_metric(g);
+  const Phys ph = to_phys(p);
+  hipStream_t s = c->stream;
+  const dim3 shapes[] = {dim3(64, 4), dim3(64, 2), dim3(128, 2), dim3(256, 1), dim3(128, 1), dim3(32, 8),
+                         dim3(64, 1)};
+  std::string rep;
+  float best = 0.f, ms = 0.f;
+  // residual faces
+  best = 1e30f;
+  for (const dim3& b : shapes) {
+    if ((rc = time_launch(c, reps, [&] {
+           k_face<double><<<grid2(m.im + 1, m.jm + 1, b), b, 0, s>>>(m, ph, gm, w, c->w0.prim, c->w0.grad, c->w0.hn);
+         }, ms)))
+      return rc;
+    BCA_LAUNCHED();
+    if (ms < best) { best = ms; c->tune_face = b; }
+  }
+  rep += "k_face " + std::to_string(c->tune_face.x) + "x" + std::to_string(c->tune_face.y) + " " +
+         std::to_string(best) + " ms; ";
+  // interior face blocks (both families share the shape)
+  best = 1e30f;
+  for (const dim3& b : shapes) {
+    BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
+    BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, nullptr, 4, m.jm + 1};
+    if ((rc = time_launch(c, reps, [&] {
+           launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
+           launch_jface_blocks_c(grid2(m.im, m.jm - 2, b), b, s, aj);
+         }, ms)))
+      return rc;
+    BCA_LAUNCHED();
+    if (ms < best) { best = ms; c->tune_blocks = b; }
+  }
+  rep += "face blocks " + std::to_string(c->tune_blocks.x) + "x" + std::to_string(c->tune_blocks.y) + " " +
+         std::to_string(best) + " ms; ";
+  // colored assembly into the caller's COO (values only)
+  best = 1e30f;
+  for (const dim3& b : shapes) {
+    dim3 gr = grid2(m.im, m.jm, b);
+    gr.z = step;
+    if ((rc = time_launch(c, reps, [&] {
+           if (z.n == 1)
+             k_assemble_colored<1><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, nullptr, jac, nullptr, nullptr, 0);
+           else
+             k_assemble_colored<2><<<gr, b, 0, s>>>(m, z, c->blki, c->blkj, nullptr, jac, nullptr, nullptr, 0);
+         }, ms)))
+      return rc;
+    BCA_LAUNCHED();
+    if (ms < best) { best = ms; c->tune_assemble = b; }
+  }
+  rep += "k_assemble_colored " + std::to_string(c->tune_assemble.x) + "x" + std::to_string(c->tune_assemble.y) +
+         " " + std::to_string(best) + " ms";
+  if (report && report_len > 0) {
+    std::strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
   return 0;
 }
 

@@ -276,6 +276,19 @@ class Flow:
                                                _ptr(indices), _ptr(data)), "bca_csr_colored_fill")
         return indptr, indices[:nnz.value], data[:nnz.value]
 
+    def autotune(self, w, jac=None, reps=3):
+        """Choose the launch shapes of the residual faces, the face blocks and
+        the colored assembly for this mesh by timing candidates on w (ghosts
+        as given); later calls on this Flow use them.  Returns the report."""
+        split = self.two_zone_split
+        nz, zp = (1, int(self.case.kind == "cyl")) if split is None else (2, split)
+        scratch = jac if jac is not None else self.t.empty(self.nbentry, dtype=self.t.float64, device="cuda")
+        buf = C.create_string_buffer(512)
+        h = self.ctx.sync_stream()
+        _lib.check(self.L.bca_autotune(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops, nz, zp,
+                                       _ptr(scratch), scratch.numel(), int(reps), buf, 512), "bca_autotune")
+        return buf.value.decode()
+
     def jacobian_csr(self, w, coefdiag=None, mini=2e-16):
         """The colored Jacobian straight into the pruned CSR
         (bca_jacobian_colored_csr): the (indptr, indices, data) that

@@ -292,6 +292,17 @@ int bca_gemm_bf16(void *hip_stream, int M, int N, int K, const unsigned short *A
                   const unsigned short *B, long sbk, long sbn, float *C, long scm, long scn, int accumulate,
                   int ksplit);
 
+/* ---- launch-shape autotuner ------------------------------------------------ */
+/* No reference counterpart.  Times candidate block shapes of the residual face
+ * kernel, the interior face-block kernels and the colored assembly on this
+ * context's mesh and state (median of reps launches each; nzones and zparam as
+ * for bca_jacobian_colored_csr; jac: a COO buffer of nbentry values used as
+ * scratch) and keeps the fastest for the context's later launches.  Results
+ * do not depend on the shapes.  report (optional) receives the choices. */
+int bca_autotune(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys, const bca_bc_op *ops,
+                 int nops, int nzones, int zparam, double *jac, long nbentry, int reps, char *report,
+                 int report_len);
+
 #ifdef __cplusplus
 }
 #endif

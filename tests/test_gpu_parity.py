@@ -375,3 +375,26 @@ def test_jacobian_graph_replays_match_eager(torch_mod, mk):
         assert t.equal(jac_g, jac_e), f"replay {rep}: COO values differ"
         assert t.equal(res_g.view(5, f.nj, f.ni)[:, 3:-3, 3:-3], res_e.view(5, f.nj, f.ni)[:, 3:-3, 3:-3])
         assert t.equal(w, we), f"replay {rep}: ghost states differ"
+
+
+@pytest.mark.parametrize("mk", [lambda: cylinder_case(44, 20, seed=71), lambda: bl_case(40, 24, seed=72)])
+def test_autotune_keeps_results(torch_mod, mk):
+    """Launch shapes chosen by bca_autotune change no value: residual and
+    colored COO before and after tuning are identical."""
+    from broadcast_amd.flow import to_device
+
+    t = torch_mod
+    case = mk()
+    f = _flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    cd = to_device(np.random.default_rng(73).uniform(0.5, 1.5, (case.im, case.jm)))
+    r0 = f.residual(w).clone()
+    j0 = f.jacobian_colored(w, coefdiag=cd)[0].clone()
+    rep = f.autotune(w)
+    print(rep, flush=True)
+    assert "k_face" in rep and "k_assemble_colored" in rep
+    r1 = f.residual(w)
+    j1 = f.jacobian_colored(w, coefdiag=cd)[0]
+    t.cuda.synchronize()
+    assert t.equal(r0, r1) and t.equal(j0, j1)
