@@ -353,6 +353,7 @@ def main():
         out["jbeta_bl512x192"] = jbeta_figure()
         out["uneven_batch_bl8"] = uneven_batch_figure()
         out["learned_newton"] = learned_figure()
+        out["residual_jvp_bl2048x1024"] = residual_jvp_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out), flush=True)
@@ -440,6 +441,54 @@ def uneven_batch_figure(reps=20):
     return {"meshes": [[f.im, f.jm] for f in flows], "cells": cells, "ms_batch": ms_batch,
             "mcells_per_s": cells / (ms_batch * 1e-3) / 1e6, "ms_per_mesh_loop": ms_loop,
             "achieved_gbs_216B_per_cell": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
+def residual_jvp_figure(im=2048, jm=1024, reps=10):
+    """BCs + residual and BCs + dense JVP (one tangent, BROADCAST_npz.py:
+    1079-1085) on a 2048x1024 boundary layer (2.1M cells, 285 MB of
+    algorithmic traffic per residual), where HBM rather than launch latency
+    sets the pace: Mcells/s and achieved GB/s at 136 B/cell (residual) and
+    216 B/cell (JVP with the primal residual)."""
+    import numpy as np
+    import torch
+    from broadcast_amd.cases import bl_case
+    from broadcast_amd.flow import Flow, to_device
+
+    case = bl_case(im, jm, seed=3000)
+    f = Flow(case)
+    w = to_device(case.w)
+    wd = to_device(np.random.default_rng(3001).standard_normal(f.shape_w))
+    res, resd = f.new_state(), f.new_state()
+    f.apply_bcs(w)
+    cells = im * jm
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def resid():
+        f.apply_bcs(w)
+        f.residual(w, res)
+
+    ms_r = timed(resid)
+    ms_j = timed(lambda: f.jvp(w, wd, res, resd))
+    out = {"cells": cells,
+           "residual": {"ms": ms_r, "mcells_per_s": cells / (ms_r * 1e-3) / 1e6,
+                        "achieved_gbs": RESIDUAL_B_PER_CELL * cells / (ms_r * 1e-3) / 1e9},
+           "jvp": {"ms": ms_j, "mcells_per_s": cells / (ms_j * 1e-3) / 1e6,
+                   "achieved_gbs": 216 * cells / (ms_j * 1e-3) / 1e9}}
+    for k in ("residual", "jvp"):
+        out[k]["frac"] = out[k]["achieved_gbs"] / HBM_PEAK_GBS
+    print(f"bench: BL {im}x{jm} residual {ms_r:.3f} ms ({out['residual']['achieved_gbs']:.0f} GB/s), "
+          f"JVP {ms_j:.3f} ms ({out['jvp']['achieved_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
+    return out
 
 
 def learned_figure():
