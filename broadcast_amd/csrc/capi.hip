@@ -1231,11 +1231,11 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
   {
     Timed t(c, "k_jac_csr_fill");
     if (z.n == 1)
-      k_jac_csr<1, true><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
-                                                   nullptr, indptr, indices, data, bad);
+      k_jac_csr_fill<1><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
+                                                  indptr, indices, data);
     else
-      k_jac_csr<2, true><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
-                                                   nullptr, indptr, indices, data, bad);
+      k_jac_csr_fill<2><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
+                                                  indptr, indices, data);
   }
   BCA_LAUNCHED();
   return 0;

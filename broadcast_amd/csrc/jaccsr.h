@@ -6,20 +6,20 @@
 // (BROADCAST_npz.py:129-135, misc/PETSc_func.py:85): per row, the slots with
 // |value| > mini, columns ascending, the slots that share a column summed.
 //
-// A block owns JC_TI consecutive i of one (j, e).  It forms the 245 * NZ slot
-// values of each of its rows in LDS from the face-local blocks -- the sum
+// The slot values are formed from the face-local blocks -- the sum
 // k_assemble_colored writes to a slot, + coefdiag on the diagonal, + the
 // ghost part (compact array of k_assemble_ghosts<true>) on rows near the
-// edge, in that order, so every value is the COO's bit for bit -- and then
+// edge, in that order, so every value is the COO's bit for bit -- in two
+// passes over tiles of JC_TI consecutive rows in i:
 //   count: marks the kept column-order indices t = (a * 7 + b) * 5 + m of each
 //          row (a, b: ranks of the slot's vali / valj among the row's
 //          distinct values), one ballot per 64 of them, and stores the kept
 //          mask and the row length;
-//   fill:  recomputes the same values and writes each row's run in CSR order
-//          (lane L of pass w takes t = 64 w + L; position indptr[row] +
-//          popcount of the mask below t), so stores are contiguous.
-// The COO's 8 B per slot write and its two reads are gone; the face blocks
-// are read once per pass.
+//   fill:  forms the same values into an LDS tile and writes each row's run
+//          in CSR order (contiguous stores).
+// The COO's 8 B per slot write and its two reads are gone, at the price of
+// forming every value twice: on the 512 x 384 O-mesh this path is ~20 %
+// slower than the COO path, and it needs none of the COO's memory.
 #pragma once
 #include "csr.h"
 #include "jacobian.h"
@@ -227,33 +227,34 @@ __global__ void __launch_bounds__(JCC_THREADS) k_jac_csr_count(Mesh m, Zones z, 
   }
 }
 
-template <int NZ, bool FILL>
-__global__ void __launch_bounds__(JC_THREADS) k_jac_csr(Mesh m, Zones z, const double* __restrict__ bi,
-                                                        const double* __restrict__ bj,
-                                                        const double* __restrict__ coefdiag,
-                                                        const double* __restrict__ G, double mini,
-                                                        unsigned long long* __restrict__ masks,
-                                                        int* __restrict__ cnt, const int* __restrict__ indptr,
-                                                        int* __restrict__ indices, double* __restrict__ data,
-                                                        int* __restrict__ bad) {
+// Fill pass.  Block = JC_TI rows of one j, all 5 variables e in turn:
+// thread (il, mm, k) keeps the face-block offsets of its row and color in
+// registers, forms the slot values of variable row e into the LDS tile, then
+// each wave writes whole rows in CSR order (lane L of pass w takes
+// column-order index t = 64 w + L; position indptr[row] + popcount of the
+// mask below t).
+template <int NZ>
+__global__ void __launch_bounds__(JC_THREADS) k_jac_csr_fill(Mesh m, Zones z, const double* __restrict__ bi,
+                                                             const double* __restrict__ bj,
+                                                             const double* __restrict__ coefdiag,
+                                                             const double* __restrict__ G, double mini,
+                                                             const unsigned long long* __restrict__ masks,
+                                                             const int* __restrict__ indptr,
+                                                             int* __restrict__ indices, double* __restrict__ data) {
   constexpr int NS = 245 * NZ;
   constexpr int PITCH = JC_TI + 1;
   constexpr int NWAVE = JC_THREADS / 64;
   constexpr int STEP = 7;  // 2 gh + 1, gh = 3 (the contexts accept no other width)
-  __shared__ double tile[FILL ? NS * PITCH : 1];
-  __shared__ JcTables<NZ, FILL> T;
+  __shared__ double tile[NS * PITCH];
+  __shared__ JcTables<NZ, true> T;
   __shared__ unsigned long long mk_s[CSR_MASK_WORDS][JC_TI];
   __shared__ int start_s[JC_TI];
-  __shared__ unsigned char flags[FILL ? 1 : JC_TI][FILL ? 1 : 64 * CSR_MASK_WORDS];
   const int tid = threadIdx.x;
   const int i0 = 1 + blockIdx.x * JC_TI, j = 1 + blockIdx.y;
   const int im = m.im, jm = m.jm;
   const long nt = 5L * im * jm;
   const int wave = tid / 64, lane = tid % 64;
-  if constexpr (!FILL) {
-    for (int x = tid; x < JC_TI * 64 * CSR_MASK_WORDS; x += JC_THREADS) (&flags[0][0])[x] = 0;
-  }
-  jc_tables<NZ, FILL>(m, z, i0, j, T);
+  jc_tables<NZ, true>(m, z, i0, j, T);
   __syncthreads();
   // item (row il, variable mm, seed color row k); its face-block offsets, the
   // column cells' zones and the row constants serve the 5 variables e
@@ -290,14 +291,12 @@ __global__ void __launch_bounds__(JC_THREADS) k_jac_csr(Mesh m, Zones z, const d
     vj = T.vjk[k];
   }
   for (int e = 0; e < 5; ++e) {
-    if constexpr (FILL) {
-      if (tid >= JC_TI * 35 && tid < JC_TI * 35 + JC_TI) {
-        const int r = tid - JC_TI * 35;
-        if (i0 + r <= im) {
-          start_s[r] = indptr[e + (j - 1) * 5 + (i0 + r - 1) * jm * 5];
+    if (tid >= JC_TI * 35 && tid < JC_TI * 35 + JC_TI) {
+      const int r = tid - JC_TI * 35;
+      if (i0 + r <= im) {
+        start_s[r] = indptr[e + (j - 1) * 5 + (i0 + r - 1) * jm * 5];
 #pragma unroll
-          for (int w = 0; w < CSR_MASK_WORDS; ++w) mk_s[w][r] = masks[w * nt + csr_tid(m, i0 + r, j, e)];
-        }
+        for (int w = 0; w < CSR_MASK_WORDS; ++w) mk_s[w][r] = masks[w * nt + csr_tid(m, i0 + r, j, e)];
       }
     }
     if (live) {
@@ -321,7 +320,6 @@ __global__ void __launch_bounds__(JC_THREADS) k_jac_csr(Mesh m, Zones z, const d
         v[D] = acc;
       }
       const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-      bool oob = false;
 #pragma unroll
       for (int l = 0; l < 7; ++l) {
         int r = l - rot;
@@ -342,62 +340,37 @@ __global__ void __launch_bounds__(JC_THREADS) k_jac_csr(Mesh m, Zones z, const d
           double val = (ok && xz == zn) ? x : 0.0;
           if (ok && coefdiag && row == c) val += cd;
           if (ok && tn >= 0) val += G[(tn * (NZ * 49) + zn * 49 + l * 7 + k) * 25 + q];
-          if constexpr (FILL) {
-            tile[(k + 7 * l + 49 * mm + 245 * zn) * PITCH + il] = val;
-          } else if (ok && fabs(val) > mini) {
-            const int a = T.posl[il][l + 7 * zn], b = T.posk[k];
-            flags[il][(a * 7 + b) * 5 + mm] = 1;
-            oob = oob || c >= nt;
-          }
+          tile[(k + 7 * l + 49 * mm + 245 * zn) * PITCH + il] = val;
         }
       }
-      if (oob) *bad = 1;
     }
     __syncthreads();
-    if constexpr (!FILL) {
-      // one ballot per mask word of each row, then the row's flags cleared
-      for (int r = wave; r < JC_TI; r += NWAVE) {
-        const int ir = i0 + r;
-        if (ir > im) break;
-        int c = 0;
-        const long tr = csr_tid(m, ir, j, e);
+    // each row's run in CSR order
+    for (int r = wave; r < JC_TI; r += NWAVE) {
+      if (i0 + r > im) break;
+      int pos = start_s[r];
 #pragma unroll
-        for (int w = 0; w < CSR_MASK_WORDS; ++w) {
-          const unsigned long long word = __ballot(flags[r][w * 64 + lane] != 0);
-          flags[r][w * 64 + lane] = 0;
-          c += __popcll(word);
-          if (lane == 0) masks[w * nt + tr] = word;
-        }
-        if (lane == 0) cnt[e + (j - 1) * 5 + (ir - 1) * jm * 5] = c;
-      }
-    } else {
-      // each row's run in CSR order
-      for (int r = wave; r < JC_TI; r += NWAVE) {
-        if (i0 + r > im) break;
-        int pos = start_s[r];
+      for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+        const unsigned long long mk = mk_s[w][r];
+        if (mk == 0) continue;
+        if ((mk >> lane) & 1ull) {
+          const int t = w * 64 + lane;
+          const int a = t / 35, b = (t / 5) % 7, m5 = t % 5;
+          const int kk = T.inv_k[b];
+          double val = 0.0;
 #pragma unroll
-        for (int w = 0; w < CSR_MASK_WORDS; ++w) {
-          const unsigned long long mk = mk_s[w][r];
-          if (mk == 0) continue;
-          if ((mk >> lane) & 1ull) {
-            const int t = w * 64 + lane;
-            const int a = t / 35, b = (t / 5) % 7, m5 = t % 5;
-            const int kk = T.inv_k[b];
-            double val = 0.0;
-#pragma unroll
-            for (int n = 0; n < NZ; ++n) {
-              const int l = T.inv_l[r][n][a];
-              if (l >= 0) {
-                const double y = tile[(kk + 7 * l + 49 * m5 + 245 * n) * PITCH + r];
-                if (fabs(y) > mini) val += y;
-              }
+          for (int n = 0; n < NZ; ++n) {
+            const int l = T.inv_l[r][n][a];
+            if (l >= 0) {
+              const double y = tile[(kk + 7 * l + 49 * m5 + 245 * n) * PITCH + r];
+              if (fabs(y) > mini) val += y;
             }
-            const int p = pos + __popcll(mk & ((1ull << lane) - 1ull));
-            indices[p] = m5 + T.col_b[b] + T.col_a[r][a];
-            data[p] = val;
           }
-          pos += __popcll(mk);
+          const int p = pos + __popcll(mk & ((1ull << lane) - 1ull));
+          indices[p] = m5 + T.col_b[b] + T.col_a[r][a];
+          data[p] = val;
         }
+        pos += __popcll(mk);
       }
     }
     __syncthreads();
