@@ -119,12 +119,20 @@ __device__ inline void face_adjoint(const Mesh& m, const double* resb, int i, in
   }
 }
 
+// The 24 seeded passes of a face are split into groups of OFF_GROUP offsets,
+// one kernel instantiation (and launch) per group: one thread evaluating all
+// 24 lets the compiler interleave them far beyond the register file (hundreds
+// of bytes of spills per lane).  Offsets [O0, O1) per instantiation.
+constexpr int OFF_GROUP = 4;
+constexpr int OFF_GROUPS = NOFF / OFF_GROUP;
+static_assert(NOFF % OFF_GROUP == 0, "offset groups");
+
 // i-faces (i, j), i in [1, im+1], rows j in [j0, j1]; O4 selects the
 // compact viscous flux (rows j >= 3) over the 2nd-order one (rows 1, 2).
 // The two extra offsets (3, 0), (-4, 0) only exist for the faces i = 1 and
 // i = im+1 (extrapolated ghost gradients); elsewhere nothing is stored for
 // them and the assembly does not read them.
-template <bool O4, int MODE>
+template <bool O4, int MODE, int O0 = 0, int O1 = NOFF>
 __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g, const double* __restrict__ w,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
@@ -136,7 +144,7 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
   const long nf = (long)(m.im + 1) * m.jm;
   FaceSink<MODE> sink{out, nf, (i - 1) + (long)(m.im + 1) * (j - 1), {}};
   if constexpr (MODE == MODE_VJP) face_adjoint(m, resb, i, j, 1, 0, sink.hb);
-  sfor<0, NOFF>([&](auto O) {
+  sfor<O0, O1>([&](auto O) {
     constexpr int o = decltype(O)::value;
     constexpr int SI = ioff_di(o), SJ = ioff_dj(o);
     if (o == 22 && i != 1) return;
@@ -152,7 +160,7 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
 // (rows j >= 4), 3, 2 = off-centred near-wall rows, 1 = the wall face.
 // Extra offsets: (0, 3) only for row 2, (0, -4) only for row jm+1; the wall
 // face depends on (0, 0) and (0, 1) only.
-template <int ROW, int MODE>
+template <int ROW, int MODE, int O0 = 0, int O1 = NOFF>
 __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g, const double* __restrict__ w,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g,
   const long nf = (long)m.im * (m.jm + 1);
   FaceSink<MODE> sink{out, nf, (i - 1) + (long)m.im * (j - 1), {}};
   if constexpr (MODE == MODE_VJP) face_adjoint(m, resb, i, j, 0, 1, sink.hb);
-  sfor<0, NOFF>([&](auto O) {
+  sfor<O0, O1>([&](auto O) {
     constexpr int o = decltype(O)::value;
     constexpr int SI = joff_di(o), SJ = joff_dj(o);
     if constexpr (o == 22 && ROW != 2) return;
