@@ -119,13 +119,14 @@ __device__ inline void face_adjoint(const Mesh& m, const double* resb, int i, in
   }
 }
 
-// The 24 seeded passes of a face are split into groups of OFF_GROUP offsets,
-// one kernel instantiation (and launch) per group: one thread evaluating all
-// 24 lets the compiler interleave them far beyond the register file (hundreds
-// of bytes of spills per lane).  Offsets [O0, O1) per instantiation.
-constexpr int OFF_GROUP = 4;
-constexpr int OFF_GROUPS = NOFF / OFF_GROUP;
-static_assert(NOFF % OFF_GROUP == 0, "offset groups");
+// The 24 seeded passes of an interior face are split into groups of
+// OFF_GROUP offsets, one kernel instantiation (and launch) per group: one
+// thread evaluating all 24 lets the compiler interleave them far beyond the
+// register file (hundreds of bytes of spills per lane).  Offsets [O0, O1) per
+// instantiation.  At 512 x 384, groups of 4 (one of six spills) take
+// 0.66 + 0.52 ms for the i- and j-face blocks, groups of 2 0.62 + 0.48 ms and
+// single offsets (no spills, 24 launches) 0.52 + 0.43 ms.
+constexpr int OFF_GROUP = 1;
 
 // i-faces (i, j), i in [1, im+1], rows j in [j0, j1]; O4 selects the
 // compact viscous flux (rows j >= 3) over the 2nd-order one (rows 1, 2).
