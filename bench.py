@@ -354,6 +354,7 @@ def main():
         out["uneven_batch_bl8"] = uneven_batch_figure()
         out["learned_newton"] = learned_figure()
         out["residual_jvp_bl2048x1024"] = residual_jvp_figure()
+        out["vjp_bl1024x256"] = vjp_figure()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
     print(json.dumps(out), flush=True)
@@ -489,6 +490,57 @@ def residual_jvp_figure(im=2048, jm=1024, reps=10):
     print(f"bench: BL {im}x{jm} residual {ms_r:.3f} ms ({out['residual']['achieved_gbs']:.0f} GB/s), "
           f"JVP {ms_j:.3f} ms ({out['jvp']['achieved_gbs']:.0f} GB/s)", file=sys.stderr, flush=True)
     return out
+
+
+def vjp_figure(im=1024, jm=256, reps=10):
+    """The adjoint wb = (dR/dw)^T resb (flux_num_dnc5_b, cylinder.py:1141) and
+    the same followed by the BC sequence's adjoint, on the 1024x256 boundary
+    layer (M = 0.5, Re_L = 3e5, perturbed Blasius state): ms per call and
+    achieved GB/s at 176 B/cell (w, metrics, resb in; wb out).  Each call
+    consumes resb's interior, as the reference's, so the timed loop copies
+    resb in first (a 10.8 MB device copy, inside the timing)."""
+    import numpy as np
+    import torch
+    from broadcast_amd.cases import bl_case
+    from broadcast_amd.flow import Flow, to_device
+
+    case = bl_case(im, jm, mach=0.5, reynolds=3e5, seed=3000)
+    f = Flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    resb0 = to_device(np.random.default_rng(3002).standard_normal(f.shape_w))
+    rb, wb = f.new_state(), f.new_state()
+    cells = im * jm
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def vjp():
+        rb.copy_(resb0)
+        f.vjp(w, rb, wb, consume=True)
+
+    def vjp_bcs():
+        rb.copy_(resb0)
+        f.vjp(w, rb, wb, consume=True)
+        f.adjoint_bcs(w, wb)
+
+    ms_v = timed(vjp)
+    ms_vb = timed(vjp_bcs)
+    gbs = 176 * cells / (ms_v * 1e-3) / 1e9
+    print(f"bench: BL {im}x{jm} VJP {ms_v:.3f} ms ({gbs:.0f} GB/s), with BC adjoints {ms_vb:.3f} ms",
+          file=sys.stderr, flush=True)
+    return {"cells": cells, "vjp_ms": ms_v, "mcells_per_s": cells / (ms_v * 1e-3) / 1e6, "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "vjp_with_bc_adjoints_ms": ms_vb,
+            "workload": "flux adjoint (24 seeded passes per face contracted with the face adjoint, fixed-order "
+                        "gather); then the BC adjoints in reverse order"}
 
 
 def learned_figure():
