@@ -1,7 +1,7 @@
 """Spanwise operators Dz, Dzz (bca_dz_5p, bca_dz_colored) and the J(beta)
 batch (bca_jbeta_*) against the reference's vectors and the oracle.
 
-FP64; operators to 1e-12 normwise per variable, COO
+FP64; operators to 1e-12 normwise per variable (the project's parity bound), COO
 indices bit-exact, J(beta) from identical COO inputs exact up to the sign of
 zeros, and at the 512x192 boundary layer with 16 wavenumbers through the
 structure J(beta) = J(0) - beta^2 Dzz + i beta Dz.

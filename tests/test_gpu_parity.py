@@ -4,7 +4,7 @@ oracle, on the same seeded inputs.
 Tolerances (FP64 everywhere): the scheme's residual is a difference of O(1)
 face fluxes, so pointwise parity is stated normwise per variable:
 max|a - b| <= tol * max|b| over the interior, tol = 1e-12 for residual, JVP
-and Jacobian rows, 1e-14 for geometry and
+and Jacobian rows (the project's parity bound), 1e-14 for geometry and
 boundary-condition ghost states.  Integer outputs (COO indices) are bit-exact.
 """
 import numpy as np
