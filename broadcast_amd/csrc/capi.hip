@@ -69,6 +69,11 @@ struct KernelTimer {
 struct bca_ctx {
   Mesh m;
   hipStream_t stream = nullptr;
+  // the edge-row face families (one or two rows of faces, a few waves each)
+  // run on these, concurrently with the interior families on `stream`; every
+  // call joins them back before it returns
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
   long bytes = 0;
   KernelTimer timer;
   Work<double> w0;
@@ -335,6 +340,11 @@ int bca_ctx_create(bca_ctx** out, int im, int jm, int gh, void* stream) {
 
 int bca_ctx_destroy(bca_ctx* c) {
   if (!c) return 0;
+  for (int k = 0; k < 3; ++k) {
+    if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+    if (c->join_ev[k]) (void)hipEventDestroy(c->join_ev[k]);
+  }
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
   free_work(c->w0);
   free_work(c->w1);
@@ -609,6 +619,54 @@ int ghost_tangents(bca_ctx* c, const double* w, const bca_metrics& g, const bca_
 
 // Phase 1 of the colored Jacobian: primitives and gradients of the state,
 // face-local blocks, ghost tangents of all zones x 49 colors.
+// Side streams of the edge-row families, created on first use outside a
+// stream capture (default priority, non-blocking).  False: run them on
+// c->stream.
+bool edge_streams(bca_ctx* c) {
+  if (c->side[0]) return true;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+  bool ok = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k)
+    ok = hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    for (int k = 0; k < 3; ++k) {
+      if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+      if (c->join_ev[k]) (void)hipEventDestroy(c->join_ev[k]);
+      c->side[k] = nullptr;
+      c->join_ev[k] = nullptr;
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    c->fork_ev = nullptr;
+    (void)hipGetLastError();
+  }
+  return ok;
+}
+
+// The streams of the edge-row families: the side streams, made to wait for
+// everything queued on c->stream so far (the primal they read), or c->stream
+// itself.
+int edge_fork(bca_ctx* c, hipStream_t es[3]) {
+  const bool par = edge_streams(c);
+  for (int k = 0; k < 3; ++k) es[k] = par ? c->side[k] : c->stream;
+  if (!par) return 0;
+  BCA_HIP(hipEventRecord(c->fork_ev, c->stream));
+  for (int k = 0; k < 3; ++k) BCA_HIP(hipStreamWaitEvent(c->side[k], c->fork_ev, 0));
+  return 0;
+}
+
+// c->stream waits for the edge-row families (before anything reads their
+// blocks, and before the call returns).
+int edge_join(bca_ctx* c, const hipStream_t es[3]) {
+  for (int k = 0; k < 3; ++k) {
+    if (es[k] == c->stream) continue;
+    BCA_HIP(hipEventRecord(c->join_ev[k], es[k]));
+    BCA_HIP(hipStreamWaitEvent(c->stream, c->join_ev[k], 0));
+  }
+  return 0;
+}
+
 int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
                     const Zones& z) {
   if (!c || !w || !p) return fail(-1, "null argument");
@@ -631,29 +689,36 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BCA_LAUNCHED();
   k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
-  // phase 1: face-local blocks (independent of the zones)
+  // phase 1: face-local blocks (independent of the zones).  The edge rows
+  // (i-face rows 1-2, j-face rows 1-3) write face rows the interior launches
+  // do not, so they run on the side streams meanwhile.
+  hipStream_t es[3];
+  if ((rc = edge_fork(c, es))) return rc;
+  BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 1, 2};
+  launch_iface_blocks_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
+  ae.out = c->blkj;
+  ae.j0 = ae.j1 = 3;
+  launch_jface_blocks_r3(grid2(m.im, 1, b), b, es[1], ae);
+  ae.j0 = ae.j1 = 2;
+  launch_jface_blocks_r2(grid2(m.im, 1, b), b, es[2], ae);
+  ae.j0 = ae.j1 = 1;
+  launch_jface_blocks_wall(grid2(m.im, 1, b), b, es[0], ae);
+  BCA_LAUNCHED();
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
   const dim3 bt = c->tune_blocks;
   {
     Timed t(c, "k_iface_blocks");
     launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, bt), bt, s, ai);
   }
-  ai.j0 = 1; ai.j1 = 2;
-  launch_iface_blocks_o2(grid2(m.im + 1, 2, b), b, s, ai);
-  BCA_LAUNCHED();
   BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, nullptr, 4, m.jm + 1};
   {
     Timed t(c, "k_jface_blocks");
     launch_jface_blocks_c(grid2(m.im, m.jm - 2, bt), bt, s, aj);
   }
-  aj.j0 = aj.j1 = 3;
-  launch_jface_blocks_r3(grid2(m.im, 1, b), b, s, aj);
-  aj.j0 = aj.j1 = 2;
-  launch_jface_blocks_r2(grid2(m.im, 1, b), b, s, aj);
-  aj.j0 = aj.j1 = 1;
-  launch_jface_blocks_wall(grid2(m.im, 1, b), b, s, aj);
   BCA_LAUNCHED();
-  return ghost_tangents(c, w, g, ops, nops, z);
+  rc = ghost_tangents(c, w, g, ops, nops, z);
+  const int rj = edge_join(c, es);
+  return rc ? rc : rj;
 }
 
 int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
@@ -734,26 +799,32 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   BCA_LAUNCHED();
   k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
+  // edge rows on the side streams while the interior runs (disjoint face
+  // rows of vjpi / vjpj; residub is only read until the join)
+  hipStream_t es[3];
+  if ((rc = edge_fork(c, es))) return rc;
+  BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 1, 2};
+  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
+  ae.out = c->vjpj;
+  ae.j0 = ae.j1 = 3;
+  launch_jface_vjp_r3(grid2(m.im, 1, b), b, es[1], ae);
+  ae.j0 = ae.j1 = 2;
+  launch_jface_vjp_r2(grid2(m.im, 1, b), b, es[2], ae);
+  ae.j0 = ae.j1 = 1;
+  launch_jface_vjp_wall(grid2(m.im, 1, b), b, es[0], ae);
+  BCA_LAUNCHED();
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 3, m.jm};
   {
     Timed t(c, "k_iface_vjp");
     launch_iface_vjp_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
   }
-  ai.j0 = 1; ai.j1 = 2;
-  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, s, ai);
-  BCA_LAUNCHED();
   BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpj, residub, 4, m.jm + 1};
   {
     Timed t(c, "k_jface_vjp");
     launch_jface_vjp_c(grid2(m.im, m.jm - 2, b), b, s, aj);
   }
-  aj.j0 = aj.j1 = 3;
-  launch_jface_vjp_r3(grid2(m.im, 1, b), b, s, aj);
-  aj.j0 = aj.j1 = 2;
-  launch_jface_vjp_r2(grid2(m.im, 1, b), b, s, aj);
-  aj.j0 = aj.j1 = 1;
-  launch_jface_vjp_wall(grid2(m.im, 1, b), b, s, aj);
   BCA_LAUNCHED();
+  if ((rc = edge_join(c, es))) return rc;
   {
     Timed t(c, "k_vjp_gather");
     k_vjp_gather<<<grid2(m.ni, m.nj, b), b, 0, s>>>(m, c->vjpi, c->vjpj, wb);
