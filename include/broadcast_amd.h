@@ -11,8 +11,12 @@
  *   face arrays  nx, ny               (ni+1) x (nj+1) x 2
  *   vol, xc, yc                       ni x nj
  *   volf                              ni x nj x 2
- * Work is enqueued on the context's HIP stream; calls return 0 on success and
- * a negative code on error (bca_last_error() gives the message).  The caller
+ * Work is enqueued on the context's HIP stream (the Jacobian and VJP calls
+ * also run their edge-row face kernels on three streams of the context that
+ * wait for it and are joined back into it before the call returns, so every
+ * result is ordered on the context's stream, also under stream capture);
+ * calls return 0 on success and a negative code on error (bca_last_error()
+ * gives the message).  The caller
  * owns every buffer; the context owns its scratch.  Calls on one context are
  * not thread-safe; separate contexts are independent.
  *
