@@ -691,7 +691,8 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BCA_LAUNCHED();
   // phase 1: face-local blocks (independent of the zones).  The edge rows
   // (i-face rows 1-2, j-face rows 1-3) write face rows the interior launches
-  // do not, so they run on the side streams meanwhile.
+  // do not, so they run on the side streams meanwhile, and so do the ghost
+  // tangents.
   hipStream_t es[3];
   if ((rc = edge_fork(c, es))) return rc;
   BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 1, 2};
@@ -704,6 +705,16 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   ae.j0 = ae.j1 = 1;
   launch_jface_blocks_wall(grid2(m.im, 1, b), b, es[0], ae);
   BCA_LAUNCHED();
+  // the ghost tangents read w only and write their own planes: on side
+  // stream 1 too (its launches go to c->stream, pointed there meanwhile)
+  const hipStream_t main = c->stream;
+  c->stream = es[1];
+  rc = ghost_tangents(c, w, g, ops, nops, z);
+  c->stream = main;
+  if (rc) {
+    (void)edge_join(c, es);
+    return rc;
+  }
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
   const dim3 bt = c->tune_blocks;
   {
@@ -716,9 +727,7 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
     launch_jface_blocks_c(grid2(m.im, m.jm - 2, bt), bt, s, aj);
   }
   BCA_LAUNCHED();
-  rc = ghost_tangents(c, w, g, ops, nops, z);
-  const int rj = edge_join(c, es);
-  return rc ? rc : rj;
+  return edge_join(c, es);
 }
 
 int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
