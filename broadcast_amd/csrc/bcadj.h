@@ -12,6 +12,7 @@
 // and the ghost tangents contracted with wb(ghosts).
 #pragma once
 #include "bcs.h"
+#include "jacobian.h"
 
 namespace bca {
 
@@ -26,11 +27,16 @@ __device__ inline void line_cell(int loc, Box b, int l, int& i, int& j, int& i0,
 }
 
 // ws = w as Dual<5> with unit tangents d[e] on variable e of the offset-d
-// cell of every line of the interface.
+// cell of every line of the interface, written on the frame the boundary
+// conditions read or write only (the ghost frame plus 4 interior layers,
+// frame_cell(m, BC_FRAME, t); the offset-d cells, d <= 3, lie in it).
 __global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ ws, int loc, Box b, int d) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m.ncell) return;
-  const int i = (int)(c % m.ni) - m.gh + 1, j = (int)(c / m.ni) - m.gh + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= frame_count(m, BC_FRAME)) return;
+  int a, bb;
+  frame_cell(m, BC_FRAME, t, a, bb);
+  const long c = a + (long)m.ni * bb;
+  const int i = a - m.gh + 1, j = bb - m.gh + 1;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const bool on = s.kdir == 1 ? (i == b.imin + d * s.i0 && j >= b.jmin && j <= b.jmax)
                               : (j == b.jmin + d * s.j0 && i >= b.imin && i <= b.imax);

@@ -1222,7 +1222,7 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
   if ((rc = alloc_planes(c, &c->wadj, 5 * m.ncell))) return rc;
   const int lmax = (int)bc_items(*op);
   for (int d = 0; d < ADJ_REACH; ++d) {
-    k_adj_seed<<<grid1(m.ncell, 256), 256, 0, c->stream>>>(m, w, c->wadj, op->loc, b, d);
+    k_adj_seed<<<grid1(frame_count(m, BC_FRAME), 256), 256, 0, c->stream>>>(m, w, c->wadj, op->loc, b, d);
     BCA_LAUNCHED();
     if ((rc = launch_bc<Dual<5>>(c, c->wadj, *op, nx, ny))) return rc;
     k_adj_accumulate<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, c->wadj, wb, op->loc, b, d, lmax);
