@@ -280,6 +280,52 @@ int bc_check(const Mesh& m, const bca_bc_op& op, const double* nx, const double*
   }
 }
 
+// Does the index box [i0, i1] x [j0, j1] lie wholly in the frame of width W
+// around the array (storage coordinates a < W or a >= ni - W, likewise b)?
+bool box_in_frame(const Mesh& m, int i0, int j0, int i1, int j1, int W) {
+  if (frame_whole(m, W)) return true;
+  const int a0 = i0 + m.gh - 1, a1 = i1 + m.gh - 1, b0 = j0 + m.gh - 1, b1 = j1 + m.gh - 1;
+  const bool meets_inner = a0 <= m.ni - W - 1 && a1 >= W && b0 <= m.nj - W - 1 && b1 >= W;
+  return !meets_inner;
+}
+
+// The BC adjoints and the Jacobian's ghost tangents seed and propagate their
+// Dual<5> states on the boundary frame only (ghost frame + 4 interior
+// layers).  A checked BC operation qualifies when every cell it reads (the
+// line and 3 cells further in, ADJ_REACH) and every ghost it writes lie in
+// that frame: the interface sits on the block edge, join boxes lie in the
+// frame.
+int frame_check(const Mesh& m, const bca_bc_op& op) {
+  const int W = BC_FRAME;
+  const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
+  if (op.kind == BCA_BC_JOIN) {
+    const Box d{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
+    if (!box_in_frame(m, b.imin, b.jmin, b.imax, b.jmax, W) || !box_in_frame(m, d.imin, d.jmin, d.imax, d.jmax, W))
+      return fail(-1, "join boxes must lie in the boundary frame (ghost layers + 4 interior layers) for the "
+                      "linearised/adjoint BC sequence of the colored Jacobian and BC adjoints");
+    return 0;
+  }
+  const Side s = decode_side(op.loc, b.imin, b.jmin, b.imax, b.jmax);
+  const int in = ADJ_REACH - 1;  // deepest interior offset read
+  int i0 = b.imin, i1 = b.imax, j0 = b.jmin, j1 = b.jmax;
+  if (s.i0 > 0) { i0 -= m.gh; i1 += in; }
+  if (s.i0 < 0) { i0 -= in; i1 += m.gh; }
+  if (s.j0 > 0) { j0 -= m.gh; j1 += in; }
+  if (s.j0 < 0) { j0 -= in; j1 += m.gh; }
+  if (!box_in_frame(m, i0, j0, i1, j1, W))
+    return fail(-1, "interface not on the block edge: the linearised/adjoint BC sequence of the colored "
+                    "Jacobian and BC adjoints covers the boundary frame (ghost layers + 4 interior layers) only");
+  return 0;
+}
+
+// bc_check + frame_check of a whole BC sequence, before anything is launched
+int check_ops_frame(const Mesh& m, const bca_bc_op* ops, int nops, const double* nx, const double* ny) {
+  int rc = 0;
+  for (int q = 0; q < nops; ++q)
+    if ((rc = bc_check(m, ops[q], nx, ny)) || (rc = frame_check(m, ops[q]))) return rc;
+  return 0;
+}
+
 // work items of a checked BC operation: lines of the interface, or cells of
 // the join box
 long bc_items(const bca_bc_op& op) {
@@ -674,6 +720,7 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
   const Mesh& m = c->m;
   int rc = 0;
+  if ((rc = check_ops_frame(m, ops, nops, g.nx, g.ny))) return rc;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
   const long nfi = (long)(m.im + 1) * m.jm, nfj = (long)m.im * (m.jm + 1);
   if ((rc = alloc_planes(c, &c->blki, (long)NOFF * 25 * nfi))) return rc;
@@ -970,7 +1017,9 @@ int bca_dz_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p
   const int step = 2 * m.gh + 1;
   if (nbentry < (long)m.im * m.jm * step * step * 25) return fail(-1, "nbentry smaller than im*jm*(2gh+1)^2*25");
   const Zones z{1, {1, 0}, {m.im, 0}, 0};
-  int rc = ghost_tangents(c, w, g, ops, nops, z);
+  int rc = check_ops_frame(m, ops, nops, g.nx, g.ny);
+  if (rc) return rc;
+  rc = ghost_tangents(c, w, g, ops, nops, z);
   if (rc) return rc;
   const dim3 b(64, 4);
   dim3 gr = grid2(m.im, m.jm, b);
@@ -1219,6 +1268,7 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
     BCA_LAUNCHED();
     return 0;
   }
+  if ((rc = frame_check(m, *op))) return rc;
   if ((rc = alloc_planes(c, &c->wadj, 5 * m.ncell))) return rc;
   const int lmax = (int)bc_items(*op);
   for (int d = 0; d < ADJ_REACH; ++d) {

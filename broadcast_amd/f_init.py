@@ -1,6 +1,6 @@
 """f_init: initialisation helpers (compilef90.py:17)."""
-from . import _lib
-from . import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def set_bndbl_2d(w, field, wbd, im, jm=None, gh=None):

@@ -352,6 +352,11 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
     ls = np.array([r["learned"]["seconds"] for r in rows])
     pi = np.array([r["plain"]["newton_steps"] for r in rows])
     li = np.array([r["learned"]["newton_steps"] for r in rows])
+    # headline: the points where both starts reached the tolerance; a plain
+    # solve stopped at its iteration cap would count at that capped time
+    both = np.array([r["plain"]["converged"] and r["learned"]["converged"] for r in rows], dtype=bool)
+    unconv = [{"re": r["re"], "mach": r["mach"], "plain_converged": r["plain"]["converged"],
+               "learned_converged": r["learned"]["converged"]} for r, b in zip(rows, both) if not b]
     return {
         "mesh": [im, jm], "train": int(len(train_p)), "train_requested": n_train, "held_out": n_test,
         "sobol_seed": seed,
@@ -361,10 +366,14 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
         "loss_first_last": [hist[0], hist[-1]],
         "training_newton_steps_mean": float(np.mean([s.iterations for s in stats])),
         "held_out_results": rows,
+        "speedup_wall_converged": float(ps[both].sum() / ls[both].sum()) if both.any() else None,
+        "converged_points": int(both.sum()), "not_converged": unconv,
+        "plain_seconds_converged": float(ps[both].sum()), "learned_seconds_converged": float(ls[both].sum()),
+        "speedup_wall_median_converged": float(np.median(ps[both] / ls[both])) if both.any() else None,
         "plain_seconds_total": float(ps.sum()), "learned_seconds_total": float(ls.sum()),
-        "speedup_wall_total": float(ps.sum() / ls.sum()),
-        "speedup_wall_median": float(np.median(ps / ls)),
-        "newton_steps_mean_plain": float(pi.mean()), "newton_steps_mean_learned": float(li.mean()),
+        "speedup_wall_total_incl_capped": float(ps.sum() / ls.sum()),
+        "newton_steps_mean_plain": float(pi[both].mean()) if both.any() else None,
+        "newton_steps_mean_learned": float(li[both].mean()) if both.any() else None,
         "all_converged": bool(all(r["plain"]["converged"] and r["learned"]["converged"] for r in rows)),
         "fallbacks": int(sum(r["learned"]["start"] != "learned" for r in rows)),
     }

@@ -3,8 +3,8 @@
 BROADCAST_npz.py:1072,1127,1245 and cylinder.py:916,978,1159."""
 from __future__ import annotations
 
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def testvector(wd, m, l, k, gh, im, jm):

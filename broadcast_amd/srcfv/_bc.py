@@ -6,8 +6,8 @@ import ctypes as C
 
 import numpy as np
 
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def inlet(w, wd, loc, interf, field, nx, ny, gam, im, jm, lm=None, gh=None):

@@ -8,9 +8,9 @@ it reads (wb += (d ghosts / d w)^T wb(ghosts)) and zeroes wb on its ghosts;
 w is left as given (the reference restores it on exit)."""
 import numpy as np
 
-from .. import _lib
-from .. import device as dev
-from . import _bc
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
+from broadcast_amd.srcfv import _bc
 
 
 def flux_num_dnc5_2d_b(res, resb, w, wb, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,

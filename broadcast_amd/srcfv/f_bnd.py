@@ -2,7 +2,7 @@
 
 Call forms of BROADCAST_npz.py:1018-1021 and handleBC.py:184-240; arrays in the
 reference's layout (numpy F-order, staged, or HIP torch tensors, zero copy)."""
-from . import _bc
+from broadcast_amd.srcfv import _bc
 
 
 def bc_supandsubinlet_2d(w, loc, interf, field, nx, ny, gam, im, jm, lm=None, gh=None):

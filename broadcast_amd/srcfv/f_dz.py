@@ -1,7 +1,7 @@
 """f_dz: spanwise operators of the 3D linearisation (srcfv/dz,
 srcfv/compile_dz.py).  Call forms of BROADCAST_npz.py:1242-1243."""
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def _dz(which, out, w, wd, nx, ny, vol, volf, gh, cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, im, jm):

@@ -1,6 +1,6 @@
 """f_geom: mesh metrics (srcfv/geom, srcfv/compile_geom.py:32)."""
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def computegeom_2d(x0, y0, nx, ny, xc, yc, vol, volf, im, jm, gh):

@@ -2,9 +2,9 @@
 
 Same names and call forms as the TAPENADE-generated _d routines the reference
 driver resolves by name (BROADCAST_npz.py:989-1001, 1079-1085)."""
-from .. import _lib
-from .. import device as dev
-from . import _bc
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
+from broadcast_amd.srcfv import _bc
 
 
 def flux_num_dnc5_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,

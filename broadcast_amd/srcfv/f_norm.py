@@ -3,8 +3,8 @@ import ctypes as C
 
 import numpy as np
 
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def _norms(fn, res, im, jm, gh):

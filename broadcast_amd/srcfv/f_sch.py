@@ -1,6 +1,6 @@
 """f_sch: the DNC residual (srcfv/rhs, srcfv/compile_rhs.py:42-44)."""
-from .. import _lib
-from .. import device as dev
+from broadcast_amd import _lib
+from broadcast_amd import device as dev
 
 
 def flux_num_dnc5_2d(res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam, rgaz, cs,

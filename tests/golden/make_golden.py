@@ -10,7 +10,8 @@ repository root:
 
 `--dz-only` adds (or refreshes) the spanwise operators dz, dz2 of the case
 fixtures from their stored inputs without regenerating anything else;
-`--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b).
+`--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b), and
+`--sbd-only` the outputs of set_bndbl_2d.
 
 Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
 """
@@ -122,6 +123,19 @@ def bcb_vectors(R, fx, seed=21):
     return out
 
 
+def sbd_vectors(R, fx):
+    """set_bndbl_2d (set_bnd.f90:2-24) on the stored state after BCs: the
+    inflow profile field(jm, gh, 5) and the top-row values wbd(im+gh, 5),
+    starting from seeded non-zero buffers (the routine overwrites them)."""
+    w = np.asfortranarray(fx["w_bc"])
+    gh, im, jm = int(fx["gh"]), int(fx["im"]), int(fx["jm"])
+    rng = np.random.default_rng(31)
+    field = np.asfortranarray(rng.standard_normal((jm, gh, 5)))
+    wbd = np.asfortranarray(rng.standard_normal((im + gh, 5)))
+    R.set_bndbl_2d(w, field, wbd, im, jm, gh)
+    return dict(sbd_field=field, sbd_wbd=wbd)
+
+
 def jac_fixture(R, case):
     g = D.geometry(R, case)
     w = np.array(case.w, order="F", copy=True)
@@ -143,11 +157,14 @@ CASES = ("bl24x16", "bl13x9", "cyl28x12")
 
 def main():
     R = oracle.Reference()
-    if "--dz-only" in sys.argv or "--bcb-only" in sys.argv:
+    only = {"--dz-only": dz_vectors, "--bcb-only": bcb_vectors, "--sbd-only": sbd_vectors}
+    extra = [fn for flag, fn in only.items() if flag in sys.argv]
+    if extra:
         for name in CASES:
             path = os.path.join(OUT, f"{name}.npz")
             fx = dict(np.load(path, allow_pickle=False))
-            fx.update(dz_vectors(R, fx) if "--dz-only" in sys.argv else bcb_vectors(R, fx))
+            for fn in extra:
+                fx.update(fn(R, fx))
             np.savez_compressed(path, **fx)
             print(f"updated {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
         return
@@ -161,6 +178,7 @@ def main():
     for name in CASES:
         fixtures[name].update(dz_vectors(R, fixtures[name]))
         fixtures[name].update(bcb_vectors(R, fixtures[name]))
+        fixtures[name].update(sbd_vectors(R, fixtures[name]))
     for name, fx in fixtures.items():
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **fx)

@@ -45,3 +45,69 @@ def test_newton_converges_and_oracle_agrees(torch_mod, mk):
     res = D.residual(O, case, g, wh)
     norm, _ = O.compute_norml2inf(res, case.im, case.jm, case.gh)
     assert np.max(norm) < 1e-10 * 1.05, norm
+
+
+def test_newton_rolls_back_a_nan_step(torch_mod, monkeypatch):
+    """Failure handling forced from the host: the second linear solve returns
+    NaN (BlockBandLU.solve patched once), so the next residual is NaN.  The
+    driver restores the state before that step bit for bit, divides cfl by
+    cfl_cut, counts one rollback, and the solve then converges.  (The
+    reference subtracts the last step again and halves cfl,
+    BROADCAST_npz.py:1170-1175; this driver restores the kept state.)"""
+    t = torch_mod
+    from broadcast_amd import newton as N
+    from broadcast_amd.flow import Flow, to_device
+
+    case = bl_case(48, 32, seed=61)
+    f = Flow(case)
+    w = to_device(case.w)
+    real = N.BlockBandLU.solve
+    calls = {"n": 0}
+    before = {}
+
+    def solve(self, b):
+        calls["n"] += 1
+        x = real(self, b)
+        if calls["n"] == 2:
+            before["w"] = w.clone()     # the state the NaN step starts from
+            return t.full_like(x, float("nan"))
+        return x
+
+    monkeypatch.setattr(N.BlockBandLU, "solve", solve)
+    # iterations: 1 step; 2 NaN step; 3 NaN residual -> rollback, stop
+    log = N.newton(f, w, ite=3, cfl=1e3, tol=1e-10, cfl_cut=10.0)
+    assert log.rollbacks == 1
+    assert log.cfl == pytest.approx(1e2)
+    assert t.equal(w, before["w"])
+    assert bool(t.isfinite(w).all())
+    log2 = N.newton(f, w, ite=40, cfl=log.cfl, tol=1e-10)
+    assert log2.converged and log2.rollbacks == 0, log2.norms
+
+
+def test_offedge_boundary_box_is_refused(torch_mod):
+    """The BC adjoints and the Jacobian's ghost tangents work on the boundary
+    frame (ghost layers + 4 interior layers): an interface away from the
+    block edge is refused with an error before any launch, not computed
+    from stale memory."""
+    import ctypes as C
+
+    from broadcast_amd import _lib
+    from broadcast_amd.flow import Flow, to_device
+
+    case = bl_case(40, 24, seed=3)
+    f = Flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    wb = f.new_state()
+    op = f._op(_lib.BC_EXTRAP2, 1, np.array([[case.im - 6, 1], [case.im - 6, case.jm]], dtype=np.float64, order="F"))
+    h = f.ctx.sync_stream()
+    rc = f.L.bca_bc_adjoint(h, C.c_void_p(w.data_ptr()), C.c_void_p(wb.data_ptr()), C.byref(op),
+                            C.c_void_p(f.nx.data_ptr()), C.c_void_p(f.ny.data_ptr()))
+    assert rc == -1
+    assert "block edge" in _lib.lib().bca_last_error().decode()
+    ops = (_lib.BcOp * 1)(op)
+    jac, ia, ja = f.jacobian_buffers()
+    rc = f.L.bca_jacobian_colored(h, C.c_void_p(w.data_ptr()), f.metrics, C.byref(f.phys), ops, 1, None, 0,
+                                  C.c_void_p(jac.data_ptr()), C.c_void_p(ia.data_ptr()), C.c_void_p(ja.data_ptr()),
+                                  f.nbentry, 1)
+    assert rc == -1

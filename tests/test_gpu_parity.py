@@ -398,3 +398,64 @@ def test_autotune_keeps_results(torch_mod, mk):
     j1 = f.jacobian_colored(w, coefdiag=cd)[0]
     t.cuda.synchronize()
     assert t.equal(r0, r1) and t.equal(j0, j1)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_norml2_and_set_bndbl_match_reference(torch_mod, name):
+    """compute_norml2 (srcfv/norm.F90:2-32) and set_bndbl_2d (set_bnd.f90:2-24)
+    through the f2py call forms (BROADCAST_npz.py:773; srcfv/f_norm.py:24), on
+    numpy arrays and on device tensors, against the reference's own outputs
+    in the fixtures: norms to 1e-13, the copies bit-exact."""
+    from broadcast_amd import f_init
+    from broadcast_amd.flow import to_device
+    from broadcast_amd.srcfv import f_norm
+
+    fx = load(name)
+    im, jm, gh = int(fx["im"]), int(fx["jm"]), int(fx["gh"])
+    for res in (np.array(fx["res"], order="F"), to_device(fx["res"])):
+        norm, nmoy = f_norm.compute_norml2(res, im, jm, gh)
+        np.testing.assert_allclose(norm, fx["norm"], rtol=1e-13)
+        np.testing.assert_allclose(nmoy, fx["nmoy"], rtol=1e-13)
+    w = np.array(fx["w_bc"], order="F")
+    field = np.full((jm, gh, 5), np.nan, order="F")
+    wbd = np.full((im + gh, 5), np.nan, order="F")
+    f_init.set_bndbl_2d(w, field, wbd, im)
+    np.testing.assert_array_equal(field, fx["sbd_field"])
+    np.testing.assert_array_equal(wbd, fx["sbd_wbd"])
+    np.testing.assert_array_equal(w, fx["w_bc"])
+
+
+def test_full_size_cylinder_csr_applies_the_jacobian(torch_mod):
+    """C1 at its configured size: the cylinder O-mesh of 512 x 384 cells (two
+    i-zones, 490 seeds).  The pruned CSR of the colored Jacobian applied to 3
+    seeded vectors u equals the pinned dense path, -(BC o R)'(w) u +
+    coefdiag u (tangent BCs + JVP), 1e-12 normwise per variable."""
+    t = torch_mod
+    from broadcast_amd.flow import Flow, to_device
+
+    case = cylinder_case(512, 384, seed=1000)
+    im, jm = case.im, case.jm
+    print(f"C1 full size {im}x{jm}", flush=True)
+    f = Flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    rng = np.random.default_rng(1001)
+    cd_np = rng.uniform(0.5, 1.5, (im, jm))
+    cd = t.tensor(cd_np.ravel(order="F"), dtype=t.float64, device="cuda")
+    jac, ia, ja = f.jacobian_colored(w, coefdiag=cd)
+    indptr, indices, data = f.csr(jac)
+    del jac, ia, ja
+    n = 5 * im * jm
+    assert int(indptr[-1]) == data.numel() and data.numel() > 100 * n
+    rows = t.repeat_interleave(t.arange(n, device="cuda"), (indptr[1:] - indptr[:-1]).long())
+    cols = indices.long()
+    for s in range(3):
+        u = rng.standard_normal((im, jm, 5))
+        ut = t.tensor(u.ravel(), dtype=t.float64, device="cuda")     # unknown e + 5 (j + jm i)
+        y = t.zeros(n, dtype=t.float64, device="cuda").index_add_(0, rows, data * ut[cols])
+        wd = np.zeros(f.shape_w, order="F")
+        wd[3:-3, 3:-3, :] = u
+        _, resd = f.jvp(w.clone(), to_device(wd))
+        ref = -_host(f, resd, f.shape_w)[3:-3, 3:-3, :] + cd_np[:, :, None] * u
+        ok, err = close(y.cpu().numpy().reshape(im, jm, 5), ref, 1e-12)
+        assert ok, f"direction {s}: CSR u vs -(BC o R)' u + coefdiag u, rel err per variable {err}"
