@@ -81,11 +81,16 @@ _SIGS = {
     "bca_bc_adjoint": [_dp, _dp, _dp, C.POINTER(BcOp), _dp, _dp],
     "bca_jacobian_colored_csr": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp, C.c_int,
                                  C.c_int, C.c_double, _dp, _dp, _dp, C.c_long, C.POINTER(C.c_long)],
+    "bca_jacobian_colored_csr_refresh": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, _dp,
+                                         C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, C.c_long,
+                                         C.POINTER(C.c_long), C.POINTER(C.c_int)],
     "bca_autotune": [_dp, _dp, Metrics, C.POINTER(Phys), C.POINTER(BcOp), C.c_int, C.c_int, C.c_int, _dp, C.c_long,
                      C.c_int, C.c_char_p, C.c_int],
     "bca_gemm_bf16": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp,
                       C.c_long, C.c_long, C.c_int, C.c_int],
 }
+
+_LONG_RESULT = ("bca_ctx_scratch_bytes",)
 
 # Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).
 EXPORTS = sorted(list(_SIGS) + ["bca_last_error"])
@@ -108,7 +113,7 @@ def lib():
         for name, argtypes in _SIGS.items():
             fn = getattr(L, name)
             fn.argtypes = argtypes
-            fn.restype = C.c_long if name == "bca_ctx_scratch_bytes" else C.c_int
+            fn.restype = C.c_long if name in _LONG_RESULT else C.c_int
         L.bca_last_error.restype = C.c_char_p
         L.bca_last_error.argtypes = []
         _lib = L

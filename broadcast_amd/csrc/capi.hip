@@ -66,6 +66,19 @@ struct KernelTimer {
   std::vector<std::pair<std::string, size_t>> marks;  // name, index of start event
 };
 
+// What the last bca_jacobian_colored_csr[_refresh] call left in the caller's
+// arrays: its zones, threshold, arrays and nnz (csr_mask / csr_cnt hold its
+// kept masks and row lengths).
+struct CsrRecord {
+  bool valid = false;
+  int nz = 0, zparam = 0;
+  double mini = 0.0;
+  const int* indptr = nullptr;
+  const int* indices = nullptr;
+  const double* data = nullptr;
+  long nnz = 0;
+};
+
 struct bca_ctx {
   Mesh m;
   hipStream_t stream = nullptr;
@@ -87,7 +100,7 @@ struct bca_ctx {
   unsigned char* ghostnz = nullptr;  // nonzero flags of the ghost tangents
   int zone_cap = 0;         // number of zones the three arrays above hold
   int* csr_cnt = nullptr;   // row lengths of the pruned CSR
-  int* csr_aux = nullptr;   // scan block sums, total, error flag
+  int* csr_aux = nullptr;   // scan block sums, then total, error flag, changed flag (CSR_AUX_INTS)
   unsigned long long* csr_mask = nullptr;  // kept masks of the rows in column order
   double* vjpi = nullptr;   // VJP contributions per face and offset, i-faces
   double* vjpj = nullptr;   // j-faces
@@ -96,6 +109,9 @@ struct bca_ctx {
   Dual<5>* wadj = nullptr;  // seeded state of the BC adjoints
   double* ghostG = nullptr; // ghost part of the fused CSR path, per near row and slot color
   long ghostG_cap = 0;
+  CsrRecord csr_rec;        // the last pruned CSR written without the COO (refresh key)
+  unsigned long long* jr_mask = nullptr;  // its kept masks (column order, 8 words per row)
+  int* jr_cnt = nullptr;                  // its row lengths
   // launch shapes chosen by bca_autotune for this mesh (defaults until then)
   dim3 tune_face{64, 4, 1};      // k_face<double>
   dim3 tune_blocks{64, 4, 1};    // interior face-block kernels of phase 1
@@ -105,6 +121,9 @@ struct bca_ctx {
 namespace {
 
 constexpr int NORM_BLOCKS = 240;
+// csr_aux: SCAN_B * SCAN_B scan block sums, then the total, the error flag
+// and the changed-row flag of the refresh pass (every user allocates this size)
+constexpr long CSR_AUX_INTS = (long)SCAN_B * SCAN_B + 4;
 
 template <class T> const char* tname();
 template <> const char* tname<double>() { return "f64"; }
@@ -403,6 +422,8 @@ int bca_ctx_destroy(bca_ctx* c) {
   for (int* p : {c->csr_cnt, c->csr_aux})
     if (p) (void)hipFree(p);
   if (c->csr_mask) (void)hipFree(c->csr_mask);
+  if (c->jr_mask) (void)hipFree(c->jr_mask);
+  if (c->jr_cnt) (void)hipFree(c->jr_cnt);
   if (c->jbeta_comp) (void)hipFree(c->jbeta_comp);
   if (c->wadj) (void)hipFree(c->wadj);
   if (c->ghostG) (void)hipFree(c->ghostG);
@@ -926,7 +947,7 @@ int bca_csr_colored_count(bca_ctx* c, const double* jac, long nbentry, int nzone
   const long rows = 5L * m.im * m.jm;
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
   if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
-  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, CSR_AUX_INTS))) return rc;
   if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
   if (nb > SCAN_B * SCAN_B) return fail(-1, "too many rows for the scan");
   int* sums = c->csr_aux;
@@ -1043,7 +1064,7 @@ int bca_jbeta_count(bca_ctx* c, const double* jac, const double* jdz, const doub
   const long rows = 5L * m.im * m.jm;
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
   if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
-  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, CSR_AUX_INTS))) return rc;
   if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
   int* sums = c->csr_aux;
   int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
@@ -1286,11 +1307,17 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
 }  // extern "C"
 
 // ------------------------------------------------------------------ fused colored Jacobian -> CSR
-extern "C" {
+namespace {
 
-int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops,
-                             int nops, const double* coefdiag, int nzones, int zparam, double mini, int* indptr,
-                             int* indices, double* data, long cap, long* nnz) {
+// The pruned CSR of the colored Jacobian without the COO (k_jac_csr_rows).
+// refresh: when the context's record says indptr / indices hold the CSR of an
+// earlier call with the same zones and mini, one row pass checks every row's
+// kept set against the stored masks and rewrites data in place; only if a row
+// changed is the CSR rescanned and refilled.  Otherwise (and always when
+// !refresh): count pass, scan, fill pass.
+int jac_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
+            const double* coefdiag, int nzones, int zparam, double mini, int* indptr, int* indices, double* data,
+            long cap, long* nnz, bool refresh, int* rebuilt) {
   if (!c || !indptr || !nnz || (cap > 0 && (!indices || !data))) return fail(-1, "null argument");
   const Mesh& m = c->m;
   const int step = 2 * m.gh + 1;
@@ -1305,6 +1332,9 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
   }
   const long rows = 5L * m.im * m.jm;
   if (rows * 245L * nzones > 2147483647L) return fail(-1, "matrix too large for 32-bit CSR indices");
+  // 32-bit face-block offsets in the row kernels
+  if ((long)(NOFF + 1) * 25 * std::max((long)(m.im + 1) * m.jm, (long)m.im * (m.jm + 1)) > 2147483647L)
+    return fail(-1, "mesh too large for the fused CSR path");
   int rc = jacobian_phase1(c, w, g, p, ops, nops, z);
   if (rc) return rc;
   hipStream_t s = c->stream;
@@ -1323,30 +1353,65 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
   BCA_LAUNCHED();
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);
   if (nb > SCAN_B * SCAN_B) return fail(-1, "too many rows for the scan");
-  if ((rc = alloc_planes(c, &c->csr_cnt, rows))) return rc;
-  if ((rc = alloc_planes(c, &c->csr_aux, (long)SCAN_B * SCAN_B + 2))) return rc;
-  if ((rc = alloc_planes(c, &c->csr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
+  // kept masks and row lengths of this path in buffers of its own: the
+  // refresh compares against them, so no other entry point may write them
+  if ((rc = alloc_planes(c, &c->jr_cnt, rows))) return rc;
+  if ((rc = alloc_planes(c, &c->csr_aux, CSR_AUX_INTS))) return rc;
+  if ((rc = alloc_planes(c, &c->jr_mask, (long)CSR_MASK_WORDS * rows))) return rc;
   int* sums = c->csr_aux;
-  int* total = c->csr_aux + (long)SCAN_B * SCAN_B;
+  int* total = c->csr_aux + (long)SCAN_B * SCAN_B;  // total, bad, changed
   int* bad = total + 1;
-  k_fill_int<<<1, 64, 0, s>>>(bad, 1, 0);
+  int* changed = total + 2;
+  k_fill_int<<<1, 64, 0, s>>>(bad, 2, 0);
   BCA_LAUNCHED();
-  // 32-bit face-block offsets in the fused kernels
-  if ((long)(NOFF + 1) * 25 * std::max((long)(m.im + 1) * m.jm, (long)m.im * (m.jm + 1)) > 2147483647L)
-    return fail(-1, "mesh too large for the fused CSR path");
   const dim3 gr((m.im + JC_TI - 1) / JC_TI, m.jm);
-  {
-    Timed t(c, "k_jac_csr_count");
-    const dim3 grc(gr.x, gr.y, 5);
-    if (z.n == 1)
-      k_jac_csr_count<1><<<grc, JCC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini,
-                                                     c->csr_mask, c->csr_cnt, bad);
-    else
-      k_jac_csr_count<2><<<grc, JCC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini,
-                                                     c->csr_mask, c->csr_cnt, bad);
+  auto rows_pass = [&](int mode, const char* name) -> int {
+    Timed t(c, name);
+#define BCA_ROWS(NZ, MODE)                                                                                     \
+  do {                                                                                                         \
+    k_jac_csr_rows<NZ, MODE, false><<<gr, JR_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG,    \
+                                                              mini, c->jr_mask, c->jr_cnt, indptr, indices,   \
+                                                              data, changed, bad);                            \
+    k_jac_csr_rows<NZ, MODE, true><<<(unsigned)jr_edge_tiles(gr.x, m.jm), JR_THREADS, 0, s>>>(                 \
+        m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->jr_mask, c->jr_cnt, indptr, indices, data, changed, \
+        bad);                                                                                                  \
+  } while (0)
+    if (z.n == 1) {
+      if (mode == JR_COUNT) BCA_ROWS(1, JR_COUNT);
+      else if (mode == JR_FILL) BCA_ROWS(1, JR_FILL);
+      else BCA_ROWS(1, JR_REFRESH);
+    } else {
+      if (mode == JR_COUNT) BCA_ROWS(2, JR_COUNT);
+      else if (mode == JR_FILL) BCA_ROWS(2, JR_FILL);
+      else BCA_ROWS(2, JR_REFRESH);
+    }
+#undef BCA_ROWS
+    BCA_LAUNCHED();
+    return 0;
+  };
+  CsrRecord& rec = c->csr_rec;
+  const bool reuse = refresh && rec.valid && rec.nz == z.n && rec.zparam == zparam && rec.mini == mini &&
+                     rec.indptr == indptr && rec.indices == indices && rec.data == data && rec.nnz <= cap;
+  rec.valid = false;
+  if (reuse) {
+    if ((rc = rows_pass(JR_REFRESH, "k_jac_csr_refresh"))) return rc;
+    int host[2] = {0, 0};
+    BCA_HIP(hipMemcpyAsync(host, bad, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    BCA_HIP(hipStreamSynchronize(s));
+    if (host[0] == 2) return fail(-4, "internal: a row outside the edge tiles is not plain");
+    if (host[0]) return fail(-4, "a kept entry has a column outside the matrix");
+    if (!host[1]) {
+      *nnz = rec.nnz;
+      *rebuilt = 0;
+      rec.valid = true;
+      return 0;
+    }
+    // some row changed: its mask and length are stored, rescan and refill
+  } else {
+    if ((rc = rows_pass(JR_COUNT, "k_jac_csr_count"))) return rc;
   }
-  BCA_LAUNCHED();
-  k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->csr_cnt, rows, indptr, sums);
+  *rebuilt = 1;
+  k_scan_blocks<<<nb, SCAN_B, 0, s>>>(c->jr_cnt, rows, indptr, sums);
   BCA_LAUNCHED();
   k_scan_sums<<<1, SCAN_B, 0, s>>>(sums, nb, total);
   BCA_LAUNCHED();
@@ -1355,20 +1420,34 @@ int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const b
   int host[2] = {0, 0};
   BCA_HIP(hipMemcpyAsync(host, total, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   BCA_HIP(hipStreamSynchronize(s));
+  if (host[1] == 2) return fail(-4, "internal: a row outside the edge tiles is not plain");
   if (host[1]) return fail(-4, "a kept entry has a column outside the matrix");
   *nnz = host[0];
   if (host[0] > cap) return fail(-5, "capacity smaller than nnz (nnz returned; call again with larger arrays)");
-  {
-    Timed t(c, "k_jac_csr_fill");
-    if (z.n == 1)
-      k_jac_csr_fill<1><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
-                                                  indptr, indices, data);
-    else
-      k_jac_csr_fill<2><<<gr, JC_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->csr_mask,
-                                                  indptr, indices, data);
-  }
-  BCA_LAUNCHED();
+  if ((rc = rows_pass(JR_FILL, "k_jac_csr_fill"))) return rc;
+  rec = CsrRecord{true, z.n, zparam, mini, indptr, indices, data, host[0]};
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bca_jacobian_colored_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops,
+                             int nops, const double* coefdiag, int nzones, int zparam, double mini, int* indptr,
+                             int* indices, double* data, long cap, long* nnz) {
+  int rebuilt = 0;
+  return jac_csr(c, w, g, p, ops, nops, coefdiag, nzones, zparam, mini, indptr, indices, data, cap, nnz, false,
+                 &rebuilt);
+}
+
+int bca_jacobian_colored_csr_refresh(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
+                                     const bca_bc_op* ops, int nops, const double* coefdiag, int nzones, int zparam,
+                                     double mini, int* indptr, int* indices, double* data, long cap, long* nnz,
+                                     int* rebuilt) {
+  if (!rebuilt) return fail(-1, "null argument");
+  return jac_csr(c, w, g, p, ops, nops, coefdiag, nzones, zparam, mini, indptr, indices, data, cap, nnz, true,
+                 rebuilt);
 }
 
 }  // extern "C"

@@ -5,29 +5,27 @@
 // and the matrix is what remove_zero_jac + scipy csr_matrix make of it
 // (BROADCAST_npz.py:129-135, misc/PETSc_func.py:85): per row, the slots with
 // |value| > mini, columns ascending, the slots that share a column summed.
+// Every value is the COO's bit for bit: the sum k_assemble_colored writes to
+// a slot (0 - left i-face + right i-face - lower j-face + upper j-face block
+// entries), + coefdiag on the diagonal, + the ghost part (compact array of
+// k_assemble_ghosts<true>) on rows near the edge, in that order.
 //
-// The slot values are formed from the face-local blocks -- the sum
-// k_assemble_colored writes to a slot, + coefdiag on the diagonal, + the
-// ghost part (compact array of k_assemble_ghosts<true>) on rows near the
-// edge, in that order, so every value is the COO's bit for bit -- in two
-// passes over tiles of JC_TI consecutive rows in i:
-//   count: marks the kept column-order indices t = (a * 7 + b) * 5 + m of each
-//          row (a, b: ranks of the slot's vali / valj among the row's
-//          distinct values), one ballot per 64 of them, and stores the kept
-//          mask and the row length;
-//   fill:  forms the same values into an LDS tile and writes each row's run
-//          in CSR order (contiguous stores).
-// The COO's 8 B per slot write and its two reads are gone, at the price of
-// forming every value twice: on the 512 x 384 O-mesh this path is ~20 %
-// slower than the COO path, and it needs none of the COO's memory.
+// k_jac_csr_rows works on tiles of JC_TI consecutive i of one j: the tile's
+// Jacobian rows of one variable e are summed in LDS from the face blocks
+// (each block entry read once per row cell that uses it, coalesced along
+// i), then each wave turns whole rows into their CSR runs.  Most rows are
+// plain (every zone's window is the 7 x 7 neighbourhood of the row cell and
+// no ghost part is added): their CSR row is the 29 stencil cells in a fixed
+// column order, handled with a lane map shared by the tile.  The other rows
+// (near the mesh edge or a zone's wrapped columns) go through the general
+// slot logic in a second launch over the edge tiles.
 #pragma once
 #include "csr.h"
 #include "jacobian.h"
 
 namespace bca {
 
-constexpr int JC_TI = 16;        // rows (consecutive i) per block
-constexpr int JC_THREADS = 576;  // 9 waves: 16 x 7 x 5 (i, k, m) value items, rounded up to waves
+constexpr int JC_TI = 16;  // rows (consecutive i) per tile
 
 // Window columns of the tile's rows, shared by all their slots: vil(i, l,
 // zone) (-1: dummy), its rank among the row's distinct values, the inverse
@@ -116,261 +114,368 @@ __device__ inline void jc_tables(const Mesh& m, const Zones& z, int i0, int j, J
   }
 }
 
-// Count pass.  Block = JC_TI rows of one (j, e), 128 threads: thread (il, k)
-// forms the slot values of seed color row k for the 5 variables m of row il
-// (5 x 28 face-block loads in flight per thread, no LDS tile), marks the kept
-// column-order indices, then one ballot per mask word and row.
-constexpr int JCC_THREADS = 128;
-template <int NZ>
-__global__ void __launch_bounds__(JCC_THREADS) k_jac_csr_count(Mesh m, Zones z, const double* __restrict__ bi,
-                                                               const double* __restrict__ bj,
-                                                               const double* __restrict__ coefdiag,
-                                                               const double* __restrict__ G, double mini,
-                                                               unsigned long long* __restrict__ masks,
-                                                               int* __restrict__ cnt, int* __restrict__ bad) {
-  constexpr int STEP = 7;  // 2 gh + 1, gh = 3 (the contexts accept no other width)
-  __shared__ JcTables<NZ, false> T;
-  __shared__ __align__(16) unsigned char flags[JC_TI][64 * CSR_MASK_WORDS];
-  const int tid = threadIdx.x;
-  const int i0 = 1 + blockIdx.x * JC_TI, j = 1 + blockIdx.y, e = blockIdx.z;
-  const int im = m.im, jm = m.jm;
-  const long nt = 5L * im * jm;
-  for (int x = tid; x < JC_TI * 64 * CSR_MASK_WORDS / 4; x += JCC_THREADS) reinterpret_cast<unsigned*>(&flags[0][0])[x] = 0u;
-  jc_tables<NZ, false>(m, z, i0, j, T);
-  __syncthreads();
-  const int il = tid % JC_TI, k = tid / JC_TI, i = i0 + il;
-  if (k < 7 && i <= im) {
-    const int nfi = (im + 1) * jm, nfj = im * (jm + 1);
-    const int fi0 = (i - 1) + (im + 1) * (j - 1), fj0 = (i - 1) + im * (j - 1);
-    const int dj = T.dj[k];
-    const bool jin = j + dj >= 1 && j + dj <= jm;
-    int b0[7], b1[7], b2[7], b3[7], zc[7];
+// ---------------------------------------------------------------- row pass
+// One kernel for the three uses of the pruned CSR without the COO; each slot
+// value is formed once per launch:
+//
+//   for each variable row e of a tile of JC_TI consecutive i (one j):
+//     J(il, k, D, m) = d(-R)/dw of the tile's row cells over their 7 x 7
+//     windows, each entry summed in registers from its (at most four) face
+//     block entries in the order of the COO path, the e-independent block
+//     offsets kept in registers from one e to the next, stored in LDS;
+//     then each wave takes whole rows and forms the kept masks by ballot
+//     (plain rows: 145 stencil candidates in three words; the others: the
+//     column-order index t = (a 7 + b) 5 + m of every slot in eight words),
+//     and
+//   JR_COUNT    stores the kept masks and row lengths;
+//   JR_FILL     writes indices and data of each kept entry at indptr[row] +
+//               popcount of the mask below it;
+//   JR_REFRESH  compares the masks with the stored ones: a row whose kept set
+//               is unchanged gets its data rewritten in place (its indptr and
+//               indices are those of the earlier call); a changed row stores
+//               its new mask and length and raises *changed (the caller then
+//               rescans and refills).
+// The mask format of a row (three words in candidate order for a plain row,
+// eight in column order otherwise) depends only on the mesh and the zones,
+// so all three modes agree on it.
+enum { JR_COUNT = 0, JR_FILL = 1, JR_REFRESH = 2 };
+constexpr int JR_THREADS = 512;
+constexpr int JR_PITCH = JC_TI + 1;
+// The cells (di, dj) relative to a row cell that its four faces' stencils
+// reach: the 29-cell stencil of the interior plus (0, 4), which the row j = 1
+// reaches through the off-centred Euler flux of the face j = 2.
+constexpr int JR_NCELL = 30;
+__host__ __device__ constexpr int jr_cell_di(int c) {
+  // rows of the table: di = -3 .. 3 with the dj values each reaches
+  return c < 1 ? -3 : c < 6 ? -2 : c < 11 ? -1 : c < 19 ? 0 : c < 24 ? 1 : c < 29 ? 2 : 3;
+}
+__host__ __device__ constexpr int jr_cell_dj(int c) {
+  return c < 1 ? 0 : c < 6 ? c - 3 : c < 11 ? c - 8 : c < 19 ? c - 14 : c < 24 ? c - 21 : c < 29 ? c - 26 : 0;
+}
+constexpr int JR_NELEM = JC_TI * JR_NCELL * 5;                   // (il, m, cell) value items of one e
+constexpr int JR_NIT = (JR_NELEM + JR_THREADS - 1) / JR_THREADS;
+
+// A row whose window is not the plain 7 x 7 neighbourhood (near the mesh
+// edge, where the ghost part is added from G when tn >= 0, or near a zone's
+// wrapped columns), in the edge launch: every slot from the tables, all
+// mask words in flight at once (this launch has a register budget of its
+// own).
+template <int NZ, int MODE>
+__device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T, const Zones& z, int r, int i,
+                                   int jm, int row, long tr, long nt, int e, double cd, bool has_cd, const double* G,
+                                   long tn, double mini, unsigned long long* masks, int* cnt, const int* indptr,
+                                   int* indices, double* data, int* changed, int* bad) {
+  constexpr int NA = 7 * NZ;
+  const int lane = threadIdx.x % 64;
+  const int rot = (i - 1) % 7;
+  double v[CSR_MASK_WORDS];
+  unsigned long long mk[CSR_MASK_WORDS];
+  bool oob = false;
 #pragma unroll
-    for (int D = 0; D < 7; ++D) {
-      const int di = D - 3;
-      const bool in = jin && i + di >= 1 && i + di <= im;
-      int o0 = ioff_index(di, dj), o1 = ioff_index(di - 1, dj);
-      int o2 = joff_index(di, dj), o3 = joff_index(di, dj - 1);
-      // base offsets of the four faces' blocks at q = 0; -1: no block
-      b0[D] = (in && o0 >= 0 && ioff_live(m, o0, i)) ? o0 * 25 * nfi + fi0 : -1;
-      b1[D] = (in && o1 >= 0 && ioff_live(m, o1, i + 1)) ? o1 * 25 * nfi + fi0 + 1 : -1;
-      b2[D] = (in && o2 >= 0 && joff_live(m, o2, j)) ? o2 * 25 * nfj + fj0 : -1;
-      b3[D] = (in && o3 >= 0 && joff_live(m, o3, j + 1)) ? o3 * 25 * nfj + fj0 + im : -1;
-      zc[D] = (NZ > 1 && i + di > z.hi[0]) ? 1 : 0;
-    }
-    const int rot = (i - 1) % STEP;
-    const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
-    const long tn = near_index(m, i, j);
-    const int vj = T.vjk[k], b = T.posk[k];
-    const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-    bool oob = false;
-#pragma unroll 1
-    for (int mm = 0; mm < 5; ++mm) {
-      const int q = e * 5 + mm;
-      double v[7];
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+    const int t = w * 64 + lane;
+    const int a = t / 35, b = (t / 5) % 7, m5 = t % 5;
+    const int kk = t < NA * 35 ? (int)T.inv_k[b] : -1;
+    double acc = 0.0;
+    bool keep = false;
+    if (kk >= 0) {
 #pragma unroll
-      for (int D = 0; D < 7; ++D) {
-        // the sum of k_assemble_colored, unguarded loads (a missing block
-        // reads offset 0) and selects
-        const double x0 = bi[(b0[D] >= 0 ? b0[D] : 0) + q * nfi], x1 = bi[(b1[D] >= 0 ? b1[D] : 0) + q * nfi];
-        const double x2 = bj[(b2[D] >= 0 ? b2[D] : 0) + q * nfj], x3 = bj[(b3[D] >= 0 ? b3[D] : 0) + q * nfj];
-        double acc = 0.0;
-        acc = b0[D] >= 0 ? acc - x0 : acc;
-        acc = b1[D] >= 0 ? acc + x1 : acc;
-        acc = b2[D] >= 0 ? acc - x2 : acc;
-        acc = b3[D] >= 0 ? acc + x3 : acc;
-        v[D] = acc;
-      }
-#pragma unroll
-      for (int l = 0; l < 7; ++l) {
-        int r = l - rot;
-        r += r < -3 ? 7 : 0;
-        r -= r > 3 ? 7 : 0;
-        double x = v[0];
-        int xz = zc[0];
-#pragma unroll
-        for (int D = 1; D < 7; ++D) {
-          x = (r == D - 3) ? v[D] : x;
-          xz = (r == D - 3) ? zc[D] : xz;
-        }
-#pragma unroll
-        for (int zn = 0; zn < NZ; ++zn) {
-          const int vi = T.vil[il][l + 7 * zn];
-          const bool ok = vi >= 0 && vj >= 0;
-          const int c = ok ? vj * 5 + vi * jm * 5 + mm : 0;
-          double val = (ok && xz == zn) ? x : 0.0;
-          if (ok && coefdiag && row == c) val += cd;
-          if (ok && tn >= 0) val += G[(tn * (NZ * 49) + zn * 49 + l * 7 + k) * 25 + q];
-          if (ok && fabs(val) > mini) {
-            flags[il][(T.posl[il][l + 7 * zn] * 7 + b) * 5 + mm] = 1;
-            oob = oob || c >= nt;
-          }
+      for (int n = 0; n < NZ; ++n) {
+        const int l = T.inv_l[r][n][a];
+        if (l < 0) continue;
+        int dl = l - rot;
+        dl += dl < -3 ? 7 : 0;
+        dl -= dl > 3 ? 7 : 0;
+        const int xz = (NZ > 1 && i + dl > z.hi[0]) ? 1 : 0;
+        const int c = T.vjk[kk] * 5 + T.vil[r][l + 7 * n] * jm * 5 + m5;
+        double y = (xz == n) ? J[((kk * 7 + dl + 3) * 5 + m5) * JR_PITCH + r] : 0.0;
+        if (has_cd && row == c) y += cd;
+        if (tn >= 0) y += G[(tn * (NZ * 49) + n * 49 + l * 7 + kk) * 25 + e * 5 + m5];
+        if (fabs(y) > mini) {
+          acc += y;
+          keep = true;
+          oob = oob || c >= nt;
         }
       }
     }
-    if (oob) *bad = 1;
+    v[w] = acc;
+    mk[w] = __ballot(keep);
   }
-  __syncthreads();
-  const int wave = tid / 64, lane = tid % 64;
-  for (int r = wave; r < JC_TI; r += JCC_THREADS / 64) {
-    const int ir = i0 + r;
-    if (ir > im) break;
-    int c = 0;
-    const long tr = csr_tid(m, ir, j, e);
+  if (oob) *bad = 1;
+  bool same = true;
+  if constexpr (MODE == JR_REFRESH) {
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) same = same && masks[w * nt + tr] == mk[w];
+  }
+  if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
+    int pos = indptr[row];
 #pragma unroll
     for (int w = 0; w < CSR_MASK_WORDS; ++w) {
-      const unsigned long long word = __ballot(flags[r][w * 64 + lane] != 0);
-      c += __popcll(word);
-      if (lane == 0) masks[w * nt + tr] = word;
+      if ((mk[w] >> lane) & 1ull) {
+        const int p = pos + __popcll(mk[w] & ((1ull << lane) - 1ull));
+        if constexpr (MODE == JR_FILL) {
+          const int t = w * 64 + lane;
+          indices[p] = t % 5 + T.col_b[(t / 5) % 7] + T.col_a[r][t / 35];
+        }
+        data[p] = v[w];
+      }
+      pos += __popcll(mk[w]);
     }
-    if (lane == 0) cnt[e + (j - 1) * 5 + (ir - 1) * jm * 5] = c;
+  } else {
+    int count = 0;
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+      count += __popcll(mk[w]);
+      if (lane == w) masks[w * nt + tr] = mk[w];
+    }
+    if (lane == 0) {
+      cnt[row] = count;
+      if (MODE == JR_REFRESH) *changed = 1;
+    }
   }
 }
 
-// Fill pass.  Block = JC_TI rows of one j, all 5 variables e in turn:
-// thread (il, mm, k) keeps the face-block offsets of its row and color in
-// registers, forms the slot values of variable row e into the LDS tile, then
-// each wave writes whole rows in CSR order (lane L of pass w takes
-// column-order index t = 64 w + L; position indptr[row] + popcount of the
-// mask below t).
-template <int NZ>
-__global__ void __launch_bounds__(JC_THREADS) k_jac_csr_fill(Mesh m, Zones z, const double* __restrict__ bi,
+// A plain row: in every zone the window is the 7 x 7 neighbourhood of the
+// row cell, and no ghost part is added.  Its CSR row is then the 29 stencil
+// cells in column order -- (di, dj) ascending, i.e. the JR cell table
+// without (0, 4) -- times the 5 variables: candidate c = 5 s + m of 145,
+// three mask words in candidate order.  A column's value is J from the zone
+// that owns its cell (the other zones' slots there are zero) and, on the
+// diagonal, coefdiag from every zone's slot: (J + cd) + cd with two zones,
+// the sums of the general path bit for bit (the kept terms are added in
+// zone order; with two kept terms the sum does not depend on the order).
+constexpr int JR_PLAIN_WORDS = 3;
+// Lane map of a plain row, the same for every plain row of one j: per mask
+// word the LDS offset of the candidate's J value (without the row) or -1,
+// and whether it is the diagonal cell (di = dj = 0) with its variable m.
+struct PlainLane {
+  int off[JR_PLAIN_WORDS];
+  int diagm[JR_PLAIN_WORDS];  // m of the diagonal cell's candidate, -1 otherwise
+};
+__device__ inline PlainLane plain_lane(int j) {
+  PlainLane pl;
+  const int lane = threadIdx.x % 64;
+#pragma unroll
+  for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
+    const int c = w * 64 + lane;
+    const int s0 = c / 5, m5 = c % 5;
+    const int s = s0 < 18 ? s0 : s0 + 1;  // skip (0, 4)
+    const int di = jr_cell_di(s), dj = jr_cell_dj(s);
+    const int k = (dj + j - 1 + 7) % 7;
+    pl.off[w] = c < 145 ? ((k * 7 + di + 3) * 5 + m5) * JR_PITCH : -1;
+    pl.diagm[w] = (c < 145 && di == 0 && dj == 0) ? m5 : -1;
+  }
+  return pl;
+}
+
+template <int NZ, int MODE>
+__device__ inline void jr_row_plain(const PlainLane& pl, const double* J, int r, int i, int j, int jm, int row,
+                                    long tr, long nt, int e, double cd, bool has_cd, double mini,
+                                    unsigned long long* masks, int* cnt, const int* indptr, int* indices,
+                                    double* data, int* changed) {
+  const int lane = threadIdx.x % 64;
+  double v[JR_PLAIN_WORDS];
+  unsigned long long mk[JR_PLAIN_WORDS];
+#pragma unroll
+  for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
+    double acc = 0.0;
+    bool keep = false;
+    if (pl.off[w] >= 0) {
+      double y = J[pl.off[w] + r];
+      if (has_cd && pl.diagm[w] == e) {
+        y += cd;
+        if (fabs(y) > mini) {
+          acc = y;
+          keep = true;
+        }
+        if (NZ > 1 && fabs(cd) > mini) {
+          acc += cd;
+          keep = true;
+        }
+      } else if (fabs(y) > mini) {
+        acc = y;
+        keep = true;
+      }
+    }
+    v[w] = acc;
+    mk[w] = __ballot(keep);
+  }
+  bool same = true;
+  if constexpr (MODE == JR_REFRESH) {
+#pragma unroll
+    for (int w = 0; w < JR_PLAIN_WORDS; ++w) same = same && masks[w * nt + tr] == mk[w];
+  }
+  if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
+    int pos = indptr[row];
+#pragma unroll
+    for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
+      if ((mk[w] >> lane) & 1ull) {
+        const int p = pos + __popcll(mk[w] & ((1ull << lane) - 1ull));
+        if constexpr (MODE == JR_FILL) {
+          const int c = w * 64 + lane;
+          const int s0 = c / 5, s = s0 < 18 ? s0 : s0 + 1;
+          indices[p] = c % 5 + (j - 1 + jr_cell_dj(s)) * 5 + (i - 1 + jr_cell_di(s)) * jm * 5;
+        }
+        data[p] = v[w];
+      }
+      pos += __popcll(mk[w]);
+    }
+  } else {
+    int count = 0;
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+      const unsigned long long word = w < JR_PLAIN_WORDS ? mk[w < JR_PLAIN_WORDS ? w : 0] : 0ull;
+      count += __popcll(word);
+      if (lane == w) masks[w * nt + tr] = word;
+    }
+    if (lane == 0) {
+      cnt[row] = count;
+      if (MODE == JR_REFRESH) *changed = 1;
+    }
+  }
+}
+
+// The edge launch: tiles that may hold rows which are not plain (the x
+// tiles of i <= 2 JC_TI and of the last two tiles, for every j, and the
+// other x tiles of the NEAR rows at each j end).  t -> (tile x, j).
+__host__ __device__ inline int jr_edge_nx(int nxt) { return nxt < 4 ? nxt : 4; }
+__host__ __device__ inline int jr_edge_nj(int jm) { return jm <= 2 * NEAR ? jm : 2 * NEAR; }
+__host__ __device__ inline long jr_edge_tiles(int nxt, int jm) {
+  return (long)jr_edge_nx(nxt) * jm + (long)(nxt - jr_edge_nx(nxt)) * jr_edge_nj(jm);
+}
+__host__ __device__ inline bool jr_edge_x(int x, int nxt) { return x < 2 || x >= nxt - 2; }
+__host__ __device__ inline bool jr_edge_j(int j, int jm) { return j <= NEAR || j > jm - NEAR; }
+__device__ inline void jr_edge_tile(long t, int nxt, int jm, int& x, int& j) {
+  const int nxe = jr_edge_nx(nxt);
+  if (t < (long)nxe * jm) {
+    const int k = (int)(t % nxe);
+    x = (nxt <= 4 || k < 2) ? k : nxt - 4 + k;
+    j = 1 + (int)(t / nxe);
+    return;
+  }
+  t -= (long)nxe * jm;
+  const int nmid = nxt - nxe;
+  x = 2 + (int)(t % nmid);
+  const int q = (int)(t / nmid);
+  j = (jm <= 2 * NEAR || q < NEAR) ? 1 + q : jm - 2 * NEAR + 1 + q;
+}
+
+template <int NZ, int MODE, bool EDGE>
+__global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, const double* __restrict__ bi,
                                                              const double* __restrict__ bj,
                                                              const double* __restrict__ coefdiag,
                                                              const double* __restrict__ G, double mini,
-                                                             const unsigned long long* __restrict__ masks,
-                                                             const int* __restrict__ indptr,
-                                                             int* __restrict__ indices, double* __restrict__ data) {
-  constexpr int NS = 245 * NZ;
-  constexpr int PITCH = JC_TI + 1;
-  constexpr int NWAVE = JC_THREADS / 64;
-  constexpr int STEP = 7;  // 2 gh + 1, gh = 3 (the contexts accept no other width)
-  __shared__ double tile[NS * PITCH];
+                                                             unsigned long long* __restrict__ masks,
+                                                             int* __restrict__ cnt, const int* __restrict__ indptr,
+                                                             int* __restrict__ indices, double* __restrict__ data,
+                                                             int* __restrict__ changed, int* __restrict__ bad) {
+  constexpr int NA = 7 * NZ;
+  constexpr int NWAVE = JR_THREADS / 64;
+  constexpr int RPW = JC_TI / NWAVE;                             // rows per wave
+  constexpr int STEP = 7;                                        // 2 gh + 1, gh = 3
+  __shared__ double J[245 * JR_PITCH];                           // (k, D, m) x il
   __shared__ JcTables<NZ, true> T;
-  __shared__ unsigned long long mk_s[CSR_MASK_WORDS][JC_TI];
-  __shared__ int start_s[JC_TI];
+  __shared__ bool plain[JC_TI];
   const int tid = threadIdx.x;
-  const int i0 = 1 + blockIdx.x * JC_TI, j = 1 + blockIdx.y;
   const int im = m.im, jm = m.jm;
+  const int nxt = (im + JC_TI - 1) / JC_TI;
+  int tx = blockIdx.x, j = 1 + blockIdx.y;
+  if constexpr (EDGE) jr_edge_tile(blockIdx.x, nxt, jm, tx, j);
+  const int i0 = 1 + tx * JC_TI;
+  const bool edge_tile = jr_edge_x(tx, nxt) || jr_edge_j(j, jm);
   const long nt = 5L * im * jm;
-  const int wave = tid / 64, lane = tid % 64;
-  jc_tables<NZ, true>(m, z, i0, j, T);
-  __syncthreads();
-  // item (row il, variable mm, seed color row k); its face-block offsets, the
-  // column cells' zones and the row constants serve the 5 variables e
-  const bool item = tid < JC_TI * 35;
-  const int il = tid % JC_TI, rest = tid / JC_TI, mm = rest % 5, k = item ? rest / 5 : 0;
-  const int i = i0 + il;
-  const bool live = item && i <= im;
   const int nfi = (im + 1) * jm, nfj = im * (jm + 1);
-  // per column offset D the offset indices o + 1 (0: no block) of the four
-  // faces' blocks, 5 bits each, and the column cell's zone in bit 20
-  int code[7];
-  double cd = 0.0;
-  long tn = -1;
-  int rot = 0, vj = -1;
-  const int fi0 = (i - 1) + (im + 1) * (j - 1), fj0 = (i - 1) + im * (j - 1);
-  if (live) {
-    const int dj = T.dj[k];
-    const bool jin = j + dj >= 1 && j + dj <= jm;
+  if (tid < 7) T.inv_k[tid] = -1;
+  jc_tables<NZ, true>(m, z, i0, j, T);
+  // window entries no face reaches stay zero for every e
+  for (int x = tid; x < 245 * JR_PITCH; x += JR_THREADS) J[x] = 0.0;
+  __syncthreads();
+  const int wave = tid / 64, lane = tid % 64;
+
+  // value items (il, m, cell): the four faces' block offsets at e = 0 (-1:
+  // no block) and the LDS slot, e-independent, kept in registers
+  int vofs[JR_NIT][4];
+  int vdst[JR_NIT];
 #pragma unroll
-    for (int D = 0; D < 7; ++D) {
-      const int di = D - 3;
-      const bool in = jin && i + di >= 1 && i + di <= im;
-      int o0 = ioff_index(di, dj), o1 = ioff_index(di - 1, dj);
-      int o2 = joff_index(di, dj), o3 = joff_index(di, dj - 1);
-      o0 = (in && o0 >= 0 && ioff_live(m, o0, i)) ? o0 + 1 : 0;
-      o1 = (in && o1 >= 0 && ioff_live(m, o1, i + 1)) ? o1 + 1 : 0;
-      o2 = (in && o2 >= 0 && joff_live(m, o2, j)) ? o2 + 1 : 0;
-      o3 = (in && o3 >= 0 && joff_live(m, o3, j + 1)) ? o3 + 1 : 0;
-      code[D] = o0 | (o1 << 5) | (o2 << 10) | (o3 << 15) | ((NZ > 1 && i + di > z.hi[0]) ? 1 << 20 : 0);
-    }
-    rot = (i - 1) % STEP;
-    cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
-    tn = near_index(m, i, j);
-    vj = T.vjk[k];
+  for (int it = 0; it < JR_NIT; ++it) {
+    const int item = tid + it * JR_THREADS;
+    const int il = item % JC_TI, rest = item / JC_TI, mm = rest % 5, cell = rest / 5;
+    const int i = i0 + il;
+    vdst[it] = -1;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) vofs[it][f] = -1;
+    if (item >= JR_NELEM || i > im) continue;
+    const int di = jr_cell_di(cell), dj = jr_cell_dj(cell);
+    const int ci = i + di, cj = j + dj;
+    const int k = ((dj + j - 1) % STEP + STEP) % STEP;
+    if (ci < 1 || ci > im || cj < 1 || cj > jm || T.dj[k] != dj) continue;
+    const int o0 = ioff_index(di, dj), o1 = ioff_index(di - 1, dj);
+    const int o2 = joff_index(di, dj), o3 = joff_index(di, dj - 1);
+    const int fi = (i - 1) + (im + 1) * (j - 1), fj = (i - 1) + im * (j - 1);
+    if (o0 >= 0 && ioff_live(m, o0, i)) vofs[it][0] = (o0 * 25 + mm) * nfi + fi;
+    if (o1 >= 0 && ioff_live(m, o1, i + 1)) vofs[it][1] = (o1 * 25 + mm) * nfi + fi + 1;
+    if (o2 >= 0 && joff_live(m, o2, j)) vofs[it][2] = (o2 * 25 + mm) * nfj + fj;
+    if (o3 >= 0 && joff_live(m, o3, j + 1)) vofs[it][3] = (o3 * 25 + mm) * nfj + fj + im;
+    vdst[it] = ((k * 7 + di + 3) * 5 + mm) * JR_PITCH + il;
   }
-  for (int e = 0; e < 5; ++e) {
-    if (tid >= JC_TI * 35 && tid < JC_TI * 35 + JC_TI) {
-      const int r = tid - JC_TI * 35;
-      if (i0 + r <= im) {
-        start_s[r] = indptr[e + (j - 1) * 5 + (i0 + r - 1) * jm * 5];
-#pragma unroll
-        for (int w = 0; w < CSR_MASK_WORDS; ++w) mk_s[w][r] = masks[w * nt + csr_tid(m, i0 + r, j, e)];
-      }
+
+  // plain rows: every zone's window is the 7 x 7 neighbourhood, no ghost part
+  if (tid < JC_TI) {
+    const int r = tid, i = i0 + r;
+    bool pl = i <= im && near_index(m, i, j) < 0;
+    const int rot = (i - 1) % STEP;
+    for (int k = 0; k < 7 && pl; ++k) pl = T.dj[k] >= -3 && T.dj[k] <= 3 && T.vjk[k] == j - 1 + T.dj[k];
+    for (int t = 0; t < NA && pl; ++t) {
+      int dl = t % 7 - rot;
+      dl += dl < -3 ? 7 : 0;
+      dl -= dl > 3 ? 7 : 0;
+      pl = T.vil[r][t] == i - 1 + dl;
     }
-    if (live) {
-      const int q = e * 5 + mm;
-      double v[7];
+    plain[r] = pl;
+    // every row that is not plain lies in a tile of the edge launch
+    if (!EDGE && !pl && i <= im && !edge_tile) *bad = 2;
+  }
+  __syncthreads();
+  const PlainLane pln = plain_lane(j);
+
+#pragma unroll 1
+  for (int e = 0; e < 5; ++e) {
+    // J(il, cell, m) of variable row e, each summed in registers in the face
+    // order of the COO path; loads from a stand-in offset 0 where a face has
+    // no block keep them all in flight, selects keep the sums exact
 #pragma unroll
-      for (int D = 0; D < 7; ++D) {
-        // jac = -dR/dw with R(i,j) = hi(i) - hi(i+1) + hj(j) - hj(j+1), the
-        // order of k_assemble_colored; unguarded loads (a missing block reads
-        // offset 0) keep all 28 in flight, the selects keep the sums exact
-        const int o0 = code[D] & 31, o1 = (code[D] >> 5) & 31, o2 = (code[D] >> 10) & 31, o3 = (code[D] >> 15) & 31;
-        const double x0 = bi[(o0 ? (o0 - 1) * 25 * nfi + fi0 : 0) + q * nfi];
-        const double x1 = bi[(o1 ? (o1 - 1) * 25 * nfi + fi0 + 1 : 0) + q * nfi];
-        const double x2 = bj[(o2 ? (o2 - 1) * 25 * nfj + fj0 : 0) + q * nfj];
-        const double x3 = bj[(o3 ? (o3 - 1) * 25 * nfj + fj0 + im : 0) + q * nfj];
-        double acc = 0.0;
-        acc = o0 ? acc - x0 : acc;
-        acc = o1 ? acc + x1 : acc;
-        acc = o2 ? acc - x2 : acc;
-        acc = o3 ? acc + x3 : acc;
-        v[D] = acc;
-      }
-      const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
-#pragma unroll
-      for (int l = 0; l < 7; ++l) {
-        int r = l - rot;
-        r += r < -3 ? 7 : 0;
-        r -= r > 3 ? 7 : 0;
-        double x = v[0];
-        int xz = code[0] >> 20;
-#pragma unroll
-        for (int D = 1; D < 7; ++D) {
-          x = (r == D - 3) ? v[D] : x;
-          xz = (r == D - 3) ? code[D] >> 20 : xz;
-        }
-#pragma unroll
-        for (int zn = 0; zn < NZ; ++zn) {
-          const int vi = T.vil[il][l + 7 * zn];
-          const bool ok = vi >= 0 && vj >= 0;
-          const int c = ok ? vj * 5 + vi * jm * 5 + mm : 0;
-          double val = (ok && xz == zn) ? x : 0.0;
-          if (ok && coefdiag && row == c) val += cd;
-          if (ok && tn >= 0) val += G[(tn * (NZ * 49) + zn * 49 + l * 7 + k) * 25 + q];
-          tile[(k + 7 * l + 49 * mm + 245 * zn) * PITCH + il] = val;
-        }
-      }
+    for (int it = 0; it < JR_NIT; ++it) {
+      if (vdst[it] < 0) continue;
+      const int h0 = vofs[it][0] >= 0, h1 = vofs[it][1] >= 0, h2 = vofs[it][2] >= 0, h3 = vofs[it][3] >= 0;
+      const double x0 = bi[h0 ? vofs[it][0] + e * 5 * nfi : 0];
+      const double x1 = bi[h1 ? vofs[it][1] + e * 5 * nfi : 0];
+      const double x2 = bj[h2 ? vofs[it][2] + e * 5 * nfj : 0];
+      const double x3 = bj[h3 ? vofs[it][3] + e * 5 * nfj : 0];
+      double acc = 0.0;
+      acc = h0 ? acc - x0 : acc;
+      acc = h1 ? acc + x1 : acc;
+      acc = h2 ? acc - x2 : acc;
+      acc = h3 ? acc + x3 : acc;
+      J[vdst[it]] = acc;
     }
     __syncthreads();
-    // each row's run in CSR order
-    for (int r = wave; r < JC_TI; r += NWAVE) {
-      if (i0 + r > im) break;
-      int pos = start_s[r];
+    // rows in column order
 #pragma unroll
-      for (int w = 0; w < CSR_MASK_WORDS; ++w) {
-        const unsigned long long mk = mk_s[w][r];
-        if (mk == 0) continue;
-        if ((mk >> lane) & 1ull) {
-          const int t = w * 64 + lane;
-          const int a = t / 35, b = (t / 5) % 7, m5 = t % 5;
-          const int kk = T.inv_k[b];
-          double val = 0.0;
-#pragma unroll
-          for (int n = 0; n < NZ; ++n) {
-            const int l = T.inv_l[r][n][a];
-            if (l >= 0) {
-              const double y = tile[(kk + 7 * l + 49 * m5 + 245 * n) * PITCH + r];
-              if (fabs(y) > mini) val += y;
-            }
-          }
-          const int p = pos + __popcll(mk & ((1ull << lane) - 1ull));
-          indices[p] = m5 + T.col_b[b] + T.col_a[r][a];
-          data[p] = val;
-        }
-        pos += __popcll(mk);
+    for (int qr = 0; qr < RPW; ++qr) {
+      const int r = wave + NWAVE * qr, i = i0 + r;
+      if (i > im) break;
+      const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
+      const long tr = csr_tid(m, i, j, e);
+      const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
+      const long tn = near_index(m, i, j);
+      if constexpr (!EDGE) {
+        if (plain[r])
+          jr_row_plain<NZ, MODE>(pln, J, r, i, j, jm, row, tr, nt, e, cd, coefdiag != nullptr, mini, masks, cnt,
+                                 indptr, indices, data, changed);
+      } else {
+        if (!plain[r])
+          jr_row_edge<NZ, MODE>(J, T, z, r, i, jm, row, tr, nt, e, cd, coefdiag != nullptr, G, tn, mini, masks,
+                                cnt, indptr, indices, data, changed, bad);
       }
     }
     __syncthreads();

@@ -289,12 +289,16 @@ class Flow:
                                        _ptr(scratch), scratch.numel(), int(reps), buf, 512), "bca_autotune")
         return buf.value.decode()
 
-    def jacobian_csr(self, w, coefdiag=None, mini=2e-16):
+    def jacobian_csr(self, w, coefdiag=None, mini=2e-16, refresh=False):
         """The colored Jacobian straight into the pruned CSR
         (bca_jacobian_colored_csr): the (indptr, indices, data) that
         csr(jacobian_colored(w, coefdiag)) gives, without forming the COO.
         The value arrays are views of buffers kept for the next call (grown
-        when a larger nnz needs it)."""
+        when a larger nnz needs it).  refresh=True (a Newton loop) goes
+        through bca_jacobian_colored_csr_refresh: where no row's kept set
+        changed since the previous call only data is rewritten; the result is
+        the same.  self.csr_rebuilt tells whether indptr/indices were
+        rewritten."""
         t = self.t
         n = 5 * self.im * self.jm
         split = self.two_zone_split
@@ -309,9 +313,18 @@ class Flow:
         for _ in range(2):
             indptr, indices, data = bufs
             h = self.ctx.sync_stream()
-            rc = self.L.bca_jacobian_colored_csr(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops,
-                                                 _ptr(coefdiag), nz, zp, float(mini), _ptr(indptr), _ptr(indices),
-                                                 _ptr(data), indices.numel(), C.byref(nnz))
+            if refresh:
+                rebuilt = C.c_int(1)
+                rc = self.L.bca_jacobian_colored_csr_refresh(
+                    h, _ptr(w), self.metrics, C.byref(self.phys), self.ops, self.nops, _ptr(coefdiag), nz, zp,
+                    float(mini), _ptr(indptr), _ptr(indices), _ptr(data), indices.numel(), C.byref(nnz),
+                    C.byref(rebuilt))
+                self.csr_rebuilt = bool(rebuilt.value)
+            else:
+                rc = self.L.bca_jacobian_colored_csr(h, _ptr(w), self.metrics, C.byref(self.phys), self.ops,
+                                                     self.nops, _ptr(coefdiag), nz, zp, float(mini), _ptr(indptr),
+                                                     _ptr(indices), _ptr(data), indices.numel(), C.byref(nnz))
+                self.csr_rebuilt = True
             if rc != -5:
                 break
             bufs = (indptr, t.empty(nnz.value, dtype=t.int32, device="cuda"),

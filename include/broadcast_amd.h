@@ -284,6 +284,18 @@ int bca_bc_adjoint(bca_ctx *ctx, const double *w, double *wb, const bca_bc_op *o
 int bca_jacobian_colored_csr(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
                              const bca_bc_op *ops, int nops, const double *coefdiag, int nzones, int zparam,
                              double mini, int *indptr, int *indices, double *data, long cap, long *nnz);
+/* The same CSR for a Newton loop (BROADCAST_npz.py:1068-1157 re-assembles
+ * and re-converts every iteration): when indptr, indices and data are the
+ * arrays the previous bca_jacobian_colored_csr[_refresh] call on this context
+ * filled, with the same nzones, zparam and mini, the kept set of every row
+ * is checked against that call's; if no row changed only data is rewritten
+ * (indptr and indices already hold this CSR) and *rebuilt = 0, otherwise the
+ * whole CSR is rebuilt as bca_jacobian_colored_csr does and *rebuilt = 1.
+ * The result is identical to bca_jacobian_colored_csr either way. */
+int bca_jacobian_colored_csr_refresh(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
+                                     const bca_bc_op *ops, int nops, const double *coefdiag, int nzones,
+                                     int zparam, double mini, int *indptr, int *indices, double *data, long cap,
+                                     long *nnz, int *rebuilt);
 
 /* ---- BF16 GEMM of the learned Newton guess ---------------------------------- */
 /* No reference counterpart (the reference has no learned part).  C (fp32,

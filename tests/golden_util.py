@@ -52,3 +52,35 @@ def close(a, b, rtol, scale=None):
             scale[1:4] = scale[1:4].max()
     err = np.max(np.abs(a - b).reshape(-1, b.shape[-1]), axis=0) if b.ndim >= 2 else np.max(np.abs(a - b))
     return np.all(err <= rtol * np.maximum(scale, 1e-300)), err / np.maximum(scale, 1e-300)
+
+
+def two_zone_diag_multiplicity(im, gh=3):
+    """How many times the two-zone relaxed assembly of an O-mesh adds coefdiag
+    to the diagonal of row cell i (1-based), per i: once per zone whose
+    window puts the diagonal's color on column i itself.  Each zone's pass
+    (computejacobianfromjv_relaxed_withjnandcheck, misc/ComputeJacobian.f90:
+    1095-1204) writes coefdiag - resd into its own slot of the doubled COO
+    whenever that slot's column is the row's own cell, and the CSR sums the
+    two.  The window column of color l for zone n follows the reference's
+    vali rules (:1121-1197); tests pin this count against the reference
+    build's scatter on small meshes."""
+    step = 2 * gh + 1
+    mult = np.zeros(im + 1, dtype=int)
+    for i in range(1, im + 1):
+        for n in (0, 1):
+            hit = False
+            for l in range(step):
+                if l <= gh:
+                    v = l if i <= l + 1 + gh else (i - (l + 1) + gh) // step * step + l
+                else:
+                    v = l if i <= l - gh else (i - (l + 1) + gh) // step * step + l
+                if n == 0:
+                    if v >= im - 2 * gh:
+                        v = l
+                elif v <= 2 * gh:
+                    v = im - im % step + l
+                    if v > im - 1:
+                        v -= step
+                hit = hit or v == i - 1
+            mult[i] += int(hit)
+    return mult[1:]

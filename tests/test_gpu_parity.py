@@ -317,6 +317,54 @@ def test_fused_jacobian_csr_equals_coo_path(torch_mod, mk):
         np.testing.assert_array_equal(a, b, err_msg=what)
 
 
+@pytest.mark.parametrize("mk", [lambda: bl_case(29, 18, seed=41),
+                                lambda: cylinder_case(30, 12, seed=42),
+                                lambda: cylinder_case(101, 40, seed=43)])
+def test_refresh_jacobian_csr_equals_coo_path(torch_mod, mk):
+    """bca_jacobian_colored_csr_refresh in a Newton-like sequence against the
+    COO path at every state, bit for bit.  The first state has rho w = 0, so
+    the entries that carry the spanwise velocity as a factor are exact zeros
+    and pruned; the same state again leaves every kept set unchanged (data
+    rewritten only); rho w perturbed makes those entries nonzero (kept sets
+    grow: the CSR is rebuilt); a perturbed state; another threshold (the
+    record no longer applies: full build)."""
+    from broadcast_amd.flow import to_device
+
+    t = torch_mod
+    case = mk()
+    print(f"case {case.kind} {case.im}x{case.jm}", flush=True)
+    f = _flow(case)
+    w = to_device(case.w)
+    wv = w.view(5, f.nj, f.ni)
+    wv[3].zero_()
+    f.apply_bcs(w)
+    rng = np.random.default_rng(44)
+    cd = to_device(rng.uniform(0.5, 1.5, (case.im, case.jm)))
+
+    def check(mini=2e-16):
+        jac, _, _ = f.jacobian_colored(w, coefdiag=cd)
+        ref = [a.cpu().numpy() for a in f.csr(jac, mini=mini)]
+        got = [a.cpu().numpy() for a in f.jacobian_csr(w, coefdiag=cd, mini=mini, refresh=True)]
+        for a, b, what in zip(got, ref, ("indptr", "indices", "data")):
+            np.testing.assert_array_equal(a, b, err_msg=what)
+        return f.csr_rebuilt, len(ref[2])
+
+    rebuilt, nnz0 = check()
+    assert rebuilt                        # no record yet: full build
+    assert check() == (False, nnz0)       # same state: values only
+    wi = wv[:, 3:-3, 3:-3]
+    wi[3] = 1e-3 * t.tensor(rng.standard_normal(tuple(wi[3].shape)), dtype=t.float64, device="cuda")
+    f.apply_bcs(w)
+    rebuilt, nnz1 = check()
+    assert rebuilt and nnz1 > nnz0        # rho w entries appear
+    assert check() == (False, nnz1)
+    wi.mul_(1.0 + 1e-3 * t.tensor(rng.standard_normal(tuple(wi.shape)), dtype=t.float64, device="cuda"))
+    f.apply_bcs(w)
+    check()                               # either way, the same CSR
+    rebuilt, _ = check(mini=1e-6)
+    assert rebuilt
+
+
 def test_two_zone_seed_and_scatter_match_oracle(torch_mod, O):
     """testvector_partial + computejacobianfromjv_relaxed_withjnandcheck
     through f_misc, bit-exact, on slots prefilled with a mix of zeros and
@@ -428,10 +476,15 @@ def test_norml2_and_set_bndbl_match_reference(torch_mod, name):
 def test_full_size_cylinder_csr_applies_the_jacobian(torch_mod):
     """C1 at its configured size: the cylinder O-mesh of 512 x 384 cells (two
     i-zones, 490 seeds).  The pruned CSR of the colored Jacobian applied to 3
-    seeded vectors u equals the pinned dense path, -(BC o R)'(w) u +
-    coefdiag u (tangent BCs + JVP), 1e-12 normwise per variable."""
+    seeded vectors u equals the pinned dense path -(BC o R)'(w) u (tangent
+    BCs + JVP), 1e-12 normwise per variable.  With the relaxation term the
+    CSR differs from it on the diagonal only, by mult * coefdiag: the
+    two-zone loop adds coefdiag once per zone whose window holds the
+    diagonal (the multiplicity pinned against the reference build's
+    scatter in the CPU suite), and the fused path gives the same CSR."""
     t = torch_mod
     from broadcast_amd.flow import Flow, to_device
+    from golden_util import two_zone_diag_multiplicity
 
     case = cylinder_case(512, 384, seed=1000)
     im, jm = case.im, case.jm
@@ -442,10 +495,15 @@ def test_full_size_cylinder_csr_applies_the_jacobian(torch_mod):
     rng = np.random.default_rng(1001)
     cd_np = rng.uniform(0.5, 1.5, (im, jm))
     cd = t.tensor(cd_np.ravel(order="F"), dtype=t.float64, device="cuda")
-    jac, ia, ja = f.jacobian_colored(w, coefdiag=cd)
-    indptr, indices, data = f.csr(jac)
-    del jac, ia, ja
     n = 5 * im * jm
+
+    def coo_csr(coefdiag):
+        jac, ia, ja = f.jacobian_colored(w, coefdiag=coefdiag)
+        out = f.csr(jac)
+        del jac, ia, ja
+        return out
+
+    indptr, indices, data = coo_csr(None)
     assert int(indptr[-1]) == data.numel() and data.numel() > 100 * n
     rows = t.repeat_interleave(t.arange(n, device="cuda"), (indptr[1:] - indptr[:-1]).long())
     cols = indices.long()
@@ -456,6 +514,22 @@ def test_full_size_cylinder_csr_applies_the_jacobian(torch_mod):
         wd = np.zeros(f.shape_w, order="F")
         wd[3:-3, 3:-3, :] = u
         _, resd = f.jvp(w.clone(), to_device(wd))
-        ref = -_host(f, resd, f.shape_w)[3:-3, 3:-3, :] + cd_np[:, :, None] * u
+        ref = -_host(f, resd, f.shape_w)[3:-3, 3:-3, :]
         ok, err = close(y.cpu().numpy().reshape(im, jm, 5), ref, 1e-12)
-        assert ok, f"direction {s}: CSR u vs -(BC o R)' u + coefdiag u, rel err per variable {err}"
+        assert ok, f"direction {s}: CSR u vs -(BC o R)' u, rel err per variable {err}"
+    # the relaxed matrix: same pattern, diagonal + mult * coefdiag
+    ip2, ix2, d2 = coo_csr(cd)
+    assert t.equal(ip2, indptr) and t.equal(ix2, indices)
+    diag = rows == cols
+    assert int(diag.sum()) == n
+    off = ~diag
+    assert t.equal(d2[off], data[off])
+    mult = two_zone_diag_multiplicity(im)                             # per i
+    want = (mult[:, None] * cd_np)[:, :, None].repeat(5, axis=2).ravel()  # per row e + 5 (j + jm i)
+    got = (d2[diag] - data[diag]).cpu().numpy()
+    scale = np.abs(d2[diag].cpu().numpy()) + np.abs(data[diag].cpu().numpy())
+    assert np.all(np.abs(got - want) <= 1e-14 * scale + 1e-15), float(np.max(np.abs(got - want) / scale))
+    # the fused path without the COO gives this CSR bit for bit
+    fused = f.jacobian_csr(w, coefdiag=cd)
+    for a, b, what in zip(fused, (ip2, ix2, d2), ("indptr", "indices", "data")):
+        assert t.equal(a, b), what

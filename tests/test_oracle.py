@@ -16,6 +16,7 @@
 import os
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import oracle
@@ -325,6 +326,42 @@ def test_two_zone_seeds_and_scatter_match_reference(O):
         outs.append((jac, ia, ja))
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("im", [17, 36, 64])
+def test_two_zone_relaxation_counts_once_per_zone(O, im):
+    """Reference quirk of the two-zone loop (cylinder.py:905-933): each zone's
+    relaxed scatter writes coefdiag - resd into its own slot whenever the
+    slot's column is the row's own cell, so the summed CSR carries coefdiag
+    once per zone whose window holds the diagonal -- twice on most rows.
+    With resd = 0 every kept entry is such a diagonal slot: the scatter of
+    all 490 seeds (the reference build where present, and the restatement)
+    gives diag(mult * coefdiag), mult from the window rules."""
+    from golden_util import two_zone_diag_multiplicity
+
+    gh, jm = 3, 9
+    ni, nj = im + 2 * gh, jm + 2 * gh
+    nb = im * jm * 49 * 25 * 2
+    split = (im // 2) // 7 * 7
+    cd = np.asfortranarray(np.random.default_rng(5).uniform(0.5, 1.5, (im, jm)))
+    resd = np.zeros((ni, nj, 5), order="F")
+    libs = [O] + ([oracle.Reference()] if os.path.exists(oracle.REF_SO) else [])
+    mult = two_zone_diag_multiplicity(im, gh)
+    assert set(mult) <= {1, 2} and (mult == 2).any()
+    want = np.repeat((mult[:, None] * cd)[:, :, None], 5, axis=2).ravel()  # rows e + 5 (j + jm i)
+    for lib in libs:
+        jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+        for n in (0, 1):
+            for m in range(5):
+                for l in range(7):
+                    for k in range(7):
+                        lib.computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, im, jm,
+                                                                         cd, 2e-16, n)
+        keep = np.abs(jac) > 2e-16
+        A = sp.csr_matrix((jac[keep], (ia[keep], ja[keep])), shape=(5 * im * jm, 5 * im * jm))
+        assert A.nnz == 5 * im * jm and np.all(A.diagonal() != 0.0)
+        np.testing.assert_allclose(A.diagonal(), want, rtol=1e-15)
+    assert split >= 7
 
 
 # ------------------------------------------------------------------ Dz, Dzz
