@@ -46,6 +46,8 @@ _SIGS = {
     "bca_geom": [_dp] * 9,
     "bca_volf": [_dp] * 3,
     "bca_residual_dnc5": [_dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_residual_dnc5_noalloc": [_dp, _dp, Metrics, C.POINTER(Phys), C.c_int, C.c_int, C.c_int, _dp],
+    "bca_residual_noalloc_bytes": [C.c_int, C.c_int, C.c_int],
     "bca_jvp_dnc5": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_vjp_dnc5": [_dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_bc_inlet": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, C.c_double],
@@ -90,7 +92,7 @@ _SIGS = {
                       C.c_long, C.c_long, C.c_int, C.c_int],
 }
 
-_LONG_RESULT = ("bca_ctx_scratch_bytes",)
+_LONG_RESULT = ("bca_ctx_scratch_bytes", "bca_residual_noalloc_bytes")
 
 # Every symbol include/broadcast_amd.h declares (checked by the CPU test suite).
 EXPORTS = sorted(list(_SIGS) + ["bca_last_error"])

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -486,6 +487,84 @@ int bca_residual_dnc5(bca_ctx* c, double* residu, const double* w, bca_metrics g
   int rc = ensure_work(c, c->w0, false);
   if (rc) return rc;
   return run_pipeline<double>(c, c->w0, w, residu, to_metric(g), to_phys(p));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- no-allocation residual
+// Scratch of bca_residual_dnc5_noalloc: the prim, grad and hn planes of the
+// residual pipeline, in one file-scope device array that every GPU the
+// library runs on holds until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in): it passes
+// the device address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) double scratch[BCA_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  BCA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(-1, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    BCA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+long bca_residual_noalloc_bytes(int im, int jm, int gh) {
+  if (gh != 3 || im < 5 || jm < 5) return -1;
+  const Mesh m = make_mesh(im, jm, gh);
+  return (long)sizeof(double) * ((NPRIM + NGRAD) * m.ncell + 10 * m.nnode);
+}
+
+int bca_residual_dnc5_noalloc(double* residu, const double* w, bca_metrics g, const bca_phys* p, int im, int jm,
+                              int gh, void* stream) {
+  if (!residu || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
+  const Mesh m = make_mesh(im, jm, gh);
+  int rc = check_mesh(m);
+  if (rc) return rc;
+  const long need = bca_residual_noalloc_bytes(im, jm, gh);
+  if (need > (long)BCA_NOALLOC_SCRATCH_BYTES)
+    return fail(-4, "bca_residual_dnc5_noalloc: the mesh needs " + std::to_string(need) +
+                        " bytes of scratch, the library's static scratch holds " +
+                        std::to_string((long)BCA_NOALLOC_SCRATCH_BYTES));
+  double* base = nullptr;
+  if ((rc = noalloc_scratch(&base))) return rc;
+  double* prim = base;
+  double* grad = prim + NPRIM * m.ncell;
+  double* hn = grad + NGRAD * m.ncell;
+  const hipStream_t s = (hipStream_t)stream;
+  const Metric mg = to_metric(g);
+  const Phys ph = to_phys(p);
+  // the kernels and launch shapes of bca_residual_dnc5 (every scratch value
+  // they read is written earlier in the same call, so nothing is cleared)
+  k_prim<double><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, prim);
+  BCA_LAUNCHED();
+  const dim3 b(64, 4);
+  k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, mg, prim, grad);
+  BCA_LAUNCHED();
+  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, grad);
+  BCA_LAUNCHED();
+  const dim3 bf(64, 4);
+  k_face<double><<<grid2(m.im + 1, m.jm + 1, bf), bf, 0, s>>>(m, ph, mg, w, prim, grad, hn);
+  BCA_LAUNCHED();
+  k_balance<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, hn, residu);
+  BCA_LAUNCHED();
+  return 0;
 }
 
 int bca_jvp_dnc5(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,

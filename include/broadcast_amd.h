@@ -94,6 +94,21 @@ int bca_volf(bca_ctx *ctx, const double *vol, double *volf);
 int bca_residual_dnc5(bca_ctx *ctx, double *residu, const double *w, bca_metrics g,
                       const bca_phys *phys);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the residual of bca_residual_dnc5, same kernels and
+ * results, without a context, enqueued on `hip_stream`.  Its scratch planes
+ * (prim, grad, hn) live in one static device array of
+ * BCA_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends; the
+ * call allocates and frees nothing.  A mesh needing more scratch
+ * (bca_residual_noalloc_bytes) is refused with -4 before any launch.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two streams, two host threads) race and must be ordered by the caller. */
+#define BCA_NOALLOC_SCRATCH_BYTES (64L * 1024L * 1024L)
+long bca_residual_noalloc_bytes(int im, int jm, int gh);
+int bca_residual_dnc5_noalloc(double *residu, const double *w, bca_metrics g, const bca_phys *phys,
+                              int im, int jm, int gh, void *hip_stream);
+
 /* ---- f_lin --------------------------------------------------------------- */
 /* flux_num_dnc5_2d_d(residu,residud,w,wd,...)
  * replaces srcfv/tangent/flux_num_dnc5_d.f90:15-3870: residu and
