@@ -4,9 +4,14 @@
 Workload (one "step"): the per-Newton-iteration work of the reference driver
 on the 2-D cylinder O-mesh of 512 (azimuthal) x 384 (radial) cells
 (M = 0.1, Re_D = 50, free stream + 1e-4 seeded noise): the boundary-condition
-sequence, one residual, and the full colored Jacobian -- 245 one-hot seeds,
-each with its linearised BCs, tangent residual and scatter into the reference's
-COO layout (BROADCAST_npz.py:1018-1127 / cylinder.py:853-980).
+sequence, one residual, and the full colored Jacobian -- 490 one-hot seeds in
+two i-zones, each equivalent to its linearised BCs and tangent residual
+(cylinder.py:895-935) -- pruned into the CSR the reference hands to PETSc
+(BROADCAST_npz.py:1018-1157).  The timed step is an iteration after the
+first: the kept pattern of every row is checked against the previous
+iteration's and the values are rewritten in place.  The reference's own
+route (its COO layout, then the prune + CSR) and the first iteration's full
+build are timed beside it.
 
 value = Mcells/s = cells per step / step time.  Inputs are resident in HBM
 before the timed region.  The dominant kernel's roofline figure comes from HIP
@@ -44,20 +49,67 @@ def kernel_ms(flow, name):
     return tot.value, n.value
 
 
-def cpu_baseline(case, budget_s=20.0):
-    """The oracle (the C++ restatement of the reference, g++ -O2, serial) on
-    the host: one residual plus a bounded sample of the seed passes of the
-    same coloring (two i-zones for an O-mesh whose im is not a multiple of
-    7), and the prune + scipy CSR of the entries they produced, scaled to a
-    full step.  (The flang build of the reference Fortran in oracle/_ref
-    computes the same numbers but runs its residual ~20x slower than the
-    restatement, so it would flatter the GPU; it is not used here.)"""
+def _seed_worker(args, keep=False):
+    """One process of the all-core CPU baseline: the oracle's seed passes
+    (testvector[_partial] + tangent BCs + JVP + scatter) of `seeds` on the
+    case, until `budget_s` of work; returns (passes done, seconds) and, with
+    keep, the COO they filled."""
+    import numpy as np
+    import oracle
+    from oracle import driver as D
+    from broadcast_amd.cases import cylinder_case
+
+    im, jm, seed, seeds, budget_s = args
+    case = cylinder_case(im, jm, seed=seed)
+    lib = oracle.Oracle()
+    g = D.geometry(lib, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(lib, case, g, w)
+    gh = case.gh
+    split = D.two_zone_split(case)
+    zones = [(0, im - 1)] if split is None else [(0, split - 1), (split, im - 1)]
+    nb = im * jm * 49 * 25 * len(zones)
+    jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+    cd = np.full((im, jm), 1e-2, order="F")
+    wd = np.zeros_like(w, order="F")
+    done, t = 0, 0.0
+    for (n, m, l, k) in seeds:
+        t1 = time.perf_counter()
+        if split is None:
+            lib.testvector(wd, m, l, k, gh, im, jm)
+        else:
+            lib.testvector_partial(wd, m, l, k, gh, im, jm, zones[n][0], zones[n][1], 0, jm - 1)
+        _, resd = D.jvp(lib, case, g, w, wd)
+        if split is None:
+            lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm, coefdiag=cd, withjn=case.kind == "cyl")
+        else:
+            lib.computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, im, jm, cd, 2e-16, n)
+        t += time.perf_counter() - t1
+        done += 1
+        if t > budget_s:
+            break
+    if keep:
+        return done, t, (jac, ia, ja)
+    return done, t
+
+
+def cpu_baseline(case, budget_s=20.0, seed=1000):
+    """The oracle (the C++ restatement of the reference, g++ -O2) on the host:
+    one BC + residual pass, a bounded sample of the seed passes of the same
+    coloring (two i-zones for an O-mesh whose im is not a multiple of 7), and
+    the prune + scipy CSR of the entries they produced, scaled to a full
+    step; on one core, and with the seed passes spread over every host core
+    the box gives this process (at most 16, one process each).  The flang
+    build of the reference Fortran computes the same numbers but runs its
+    residual ~20x slower than the restatement, so it is not the timed
+    baseline.  Runs before the GPU is touched (the worker processes are
+    forked)."""
+    import multiprocessing as mp
     import numpy as np
     import scipy.sparse as sp
     import oracle
     from oracle import driver as D
 
-    kind = "port"
     lib = oracle.Oracle()
     g = D.geometry(lib, case)
     w = np.array(case.w, order="F", copy=True)
@@ -67,51 +119,47 @@ def cpu_baseline(case, budget_s=20.0):
     t_res = time.perf_counter() - t0
     im, jm, gh = case.im, case.jm, case.gh
     split = D.two_zone_split(case)
-    zones = [(0, im - 1)] if split is None else [(0, split - 1), (split, im - 1)]
-    nb = im * jm * 49 * 25 * len(zones)
-    jac, ia, ja = np.zeros(nb), np.zeros(nb, np.int32), np.zeros(nb, np.int32)
-    cd = np.full((im, jm), 1e-2, order="F")
-    wd = np.zeros_like(w, order="F")
-    seeds = [(n, m, l, k) for n in range(len(zones)) for m in range(5) for l in range(7) for k in range(7)]
-    done, t_seed = 0, 0.0
-    for (n, m, l, k) in seeds:
-        t1 = time.perf_counter()
-        if split is None:
-            lib.testvector(wd, m, l, k, gh, im, jm)
-        else:
-            lib.testvector_partial(wd, m, l, k, gh, im, jm, zones[n][0], zones[n][1], 0, jm - 1)
-        _, resd = D.jvp(lib, case, g, w, wd)
-        if split is None:
-            lib.computejacobianfromjv(jac, ia, ja, resd, m, l, k, gh, im, jm, coefdiag=cd,
-                                      withjn=case.kind == "cyl")
-        else:
-            lib.computejacobianfromjv_relaxed_withjnandcheck(jac, ia, ja, resd, m, l, k, gh, im, jm, cd, 2e-16, n)
-        t_seed += time.perf_counter() - t1
-        done += 1
-        if t_res + t_seed > budget_s:
-            break
-    # prune + CSR of the slots the sampled seeds filled (remove_zero_jac,
-    # csr_matrix), scaled by the fraction of slots
-    filled = np.zeros(nb, bool)
+    nz = 1 if split is None else 2
+    seeds = [(n, m, l, k) for n in range(nz) for m in range(5) for l in range(7) for k in range(7)]
+    # one core
+    done1, t_seed1, (jac, ia, ja) = _seed_worker((im, jm, seed, seeds, budget_s * 0.5), keep=True)
+    per1 = t_seed1 / done1
+    # prune + CSR of the slots the sampled seeds filled, scaled by the slot fraction
     per = im * jm * 5
-    for (n, m, l, k) in seeds[:done]:
-        s0 = (k * im * jm * 5 + l * im * jm * 5 * 7 + m * im * jm * 5 * 49 + n * im * jm * 5 * 245)
+    filled = np.zeros(jac.size, bool)
+    for (n, m, l, k) in seeds[:done1]:
+        s0 = k * per + l * per * 7 + m * per * 49 + n * per * 245
         filled[s0:s0 + per] = True
     t2 = time.perf_counter()
     sub = np.flatnonzero(filled)
     keep = np.abs(jac[sub]) > 2e-16
     sp.csr_matrix((jac[sub][keep], (ia[sub][keep], ja[sub][keep])), shape=(per, per))
-    t_csr = (time.perf_counter() - t2) * len(seeds) / done
-    step_s = t_res + t_seed / done * len(seeds) + t_csr
+    t_csr = (time.perf_counter() - t2) * len(seeds) / done1
+    del jac, ia, ja, filled, sub, keep
+    step1 = t_res + per1 * len(seeds) + t_csr
+    # all cores: each process a share of the seed list, the same work budget
+    ncores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    shares = [seeds[q::ncores] for q in range(ncores)]
+    ctx = mp.get_context("fork")
+    t3 = time.perf_counter()
+    with ctx.Pool(ncores) as pool:
+        res = pool.map(_seed_worker, [(im, jm, seed, sh, budget_s * 0.5) for sh in shares])
+    wall = time.perf_counter() - t3
+    doneN = sum(d for d, _ in res)
+    # aggregate seed-pass rate with every process busy (each ran until its budget)
+    rateN = sum(d / t for d, t in res)
+    stepN = t_res + len(seeds) / rateN + t_csr
     return {
-        "value": im * jm / step_s / 1e6,
+        "value": im * jm / stepN / 1e6,
         "unit": "Mcells/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": f"1 BC+residual and {done} of {len(seeds)} seed passes (testvector + tangent BCs + JVP + "
-                  f"scatter) and the prune + scipy CSR of their entries, on the {im}x{jm} O-mesh, serial, "
-                  f"scaled to a full step ({t_res:.2f} s residual, {t_seed / done:.2f} s per seed, "
-                  f"{t_csr:.2f} s CSR)",
+        "cores": ncores,
+        "kind": "port",
+        "sample": f"{im}x{jm} O-mesh, {len(seeds)}-seed step: 1 BC+residual ({t_res:.2f} s), {doneN} seed passes "
+                  f"(testvector + tangent BCs + JVP + scatter) over {ncores} processes in {wall:.1f} s "
+                  f"(aggregate {rateN:.2f} passes/s), prune + scipy CSR scaled from {done1} passes "
+                  f"({t_csr:.2f} s); scaled to a full step of {stepN:.1f} s",
+        "single_core": {"value": im * jm / step1 / 1e6, "unit": "Mcells/s", "cores": 1,
+                        "sample": f"{done1} of {len(seeds)} seed passes, {per1:.2f} s each; full step {step1:.1f} s"},
         "host_cpu": _cpu_model(),
         "nproc": os.cpu_count(),
     }
@@ -151,6 +199,12 @@ def main():
     from broadcast_amd.cases import cylinder_case
     from broadcast_amd.flow import Flow, to_device
 
+    cpu = None
+    if not args.no_cpu_baseline:
+        # before the GPU is touched: the all-core leg forks worker processes
+        print("bench: CPU baseline (oracle, 1 core and all cores)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(cylinder_case(args.im, args.jm, seed=1000 * 1 + 0), args.cpu_budget)
+        print(f"bench: CPU baseline {cpu['value']:.4f} Mcells/s on {cpu['cores']} cores", file=sys.stderr, flush=True)
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -178,39 +232,44 @@ def main():
     tuned = f.autotune(w, jac=bufs[0])
     print(f"bench: autotuned {tuned}", file=sys.stderr, flush=True)
 
-    def step(first=False):
+    def step():
+        # the Newton loop's Jacobian: straight into the pruned CSR; the kept
+        # pattern of every row is checked and, while it holds, only the
+        # values are rewritten (bca_jacobian_colored_csr_refresh)
+        f.apply_bcs(w)
+        f.residual(w, res)
+        return f.jacobian_csr(w, coefdiag=cd, refresh=True)
+
+    def step_coo(first=False):
+        # the reference's own route: the COO of its layout, then the prune + CSR
         f.apply_bcs(w)
         f.residual(w, res)
         f.jacobian_colored(w, coefdiag=cd, bufs=bufs, write_index=first)
         return f.csr(bufs[0])
 
-    def step_fused(first=False):
-        # the same matrix straight into the CSR, without the COO
-        f.apply_bcs(w)
-        f.residual(w, res)
-        return f.jacobian_csr(w, coefdiag=cd)
+    def timed(fn, n, names):
+        fn()
+        torch.cuda.synchronize()
+        _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ks = {}
+        for name in names:
+            tot, cnt = kernel_ms(f, name)
+            ks[name] = {"total_ms": tot, "launches": cnt, "avg_ms": tot / max(cnt, 1)}
+        return e0.elapsed_time(e1) / n, ks
 
     print(f"bench: {case.name} cells={cells} warmup={args.warmup} steps={args.steps}", file=sys.stderr,
           flush=True)
-    nnz = int(step(first=True)[1].numel())
-    for _ in range(max(args.warmup - 1, 0)):
-        step()
-    step_fused(first=True)
-    torch.cuda.synchronize()
-    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
-    ec0, ec1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ec0.record()
-    for _ in range(max(args.steps, 1)):
-        step_fused()
-    ec1.record()
-    torch.cuda.synchronize()
-    fused_ms = ec0.elapsed_time(ec1) / max(args.steps, 1)
-    fused_kernels = {}
-    for name in ("k_jac_csr_count", "k_jac_csr_fill", "k_assemble_ghosts"):
-        tot, n = kernel_ms(f, name)
-        fused_kernels[name] = {"total_ms": tot, "launches": n, "avg_ms": tot / max(n, 1)}
-    f._csr_bufs = None  # release the fused path's CSR buffers
-    # the Jacobian stage replayed from a HIP graph, then the pruned CSR
+    # the COO route (figure beside the headline), then release its buffers
+    nnz_coo = int(step_coo(first=True)[1].numel())
+    coo_ms, coo_kernels = timed(step_coo, max(args.steps, 1),
+                                ("k_iface_blocks", "k_jface_blocks", "ghost_tangents", "k_assemble_colored",
+                                 "k_assemble_ghosts", "k_csr_count", "k_csr_fill"))
     from broadcast_amd.graph import JacobianGraph
 
     G = JacobianGraph(f, w, coefdiag=cd, bufs=bufs)
@@ -225,72 +284,71 @@ def main():
     eg1.record()
     torch.cuda.synchronize()
     graph_ms = eg0.elapsed_time(eg1) / max(args.steps, 1)
-    del G
+    del G, bufs
+    torch.cuda.empty_cache()
 
-    # residual alone (same kernels as inside the step), for its own figure
-    nres = 20
-    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(nres):
+    # the full build without the COO (count, scan, fill), as a first Newton
+    # iteration does it
+    def step_build():
         f.apply_bcs(w)
         f.residual(w, res)
-    ev1.record()
-    torch.cuda.synchronize()
-    res_ms = ev0.elapsed_time(ev1) / nres
-    face64_ms, face64_n = kernel_ms(f, "k_face<f64>")
+        return f.jacobian_csr(w, coefdiag=cd)
 
-    # timed region
-    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
+    build_ms, build_kernels = timed(step_build, max(args.steps // 2, 1),
+                                    ("k_jac_csr_count", "k_jac_csr_fill", "k_assemble_ghosts"))
+
+    # residual alone (same kernels as inside the step), for its own figure
+    res_ms, res_kernels = timed(lambda: (f.apply_bcs(w), f.residual(w, res)), 20, ("k_face<f64>",))
+    face64 = res_kernels["k_face<f64>"]
+
+    # timed region: the headline step
+    names = ("k_iface_blocks", "k_jface_blocks", "ghost_tangents", "k_assemble_ghosts", "k_jac_csr_refresh")
+    for _ in range(max(args.warmup, 1)):
+        step()
     torch.cuda.synchronize()
+    _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 1), "bca_profile_enable")
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
     for _ in range(args.steps):
-        step()
+        indptr, indices, data = step()
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
+    assert not f.csr_rebuilt, "the kept pattern changed inside the timed region"
+    nnz = int(data.numel())
+    assert nnz == nnz_coo, (nnz, nnz_coo)
     kernels = {}
-    for name in ("k_iface_blocks", "k_jface_blocks", "ghost_tangents", "k_assemble_colored", "k_assemble_ghosts",
-                 "k_csr_count", "k_csr_fill"):
+    for name in names:
         tot, n = kernel_ms(f, name)
         kernels[name] = {"total_ms": tot, "launches": n, "avg_ms": tot / max(n, 1)}
     _lib.check(_lib.lib().bca_profile_enable(f.ctx.h, 0), "bca_profile_enable")
     print(f"bench: {ms_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f}) kernels {kernels}",
           file=sys.stderr, flush=True)
 
-    # Algorithmic bytes of the step's Jacobian stage: 8 B per CSR value
-    # written + 96 B per cell of shared inputs (w and metrics); the COO of the
-    # reference layout is scratch and the CSR indices are a per-mesh pattern,
-    # so neither counts.  The roofline line gives these bytes to the
-    # dominant kernel.  The per-kernel figures below also count the COO and
-    # face-block traffic each kernel moves by design (coo_inclusive).
+    # Algorithmic bytes of the step (SURVEY 8(d) for the colored Jacobian):
+    # 8 B per CSR value written + 96 B per cell of shared inputs (w and
+    # metrics); the face blocks are scratch and the CSR indices a per-mesh
+    # pattern.  The roofline line gives these bytes to the dominant kernel.
     step_alg = 8 * nnz + JVP_SHARED_B_PER_CELL * cells
     nfi, nfj = (case.im + 1) * case.jm, case.im * (case.jm + 1)
-    blocks_read = 8 * 25 * 22 * (nfi + nfj)  # the 22 interior-stencil face blocks of 5 x 5 doubles
-    alg_coo = {
-        # state + metrics in (96 B/cell) and the 24 x 5 x 5 face-local blocks out
-        "k_iface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfi - 2 * (case.im + 1)),
-        "k_jface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * (nfj - 3 * case.im),
-        # Jacobian values written in the reference's COO layout (8 B each) and
-        # the face blocks read once per face
-        "k_assemble_colored": 8 * f.nbentry + blocks_read,
-        # prune + CSR: the COO values read (count; the kept masks, 64 B per
-        # row, written), read again with the CSR indices and values written
-        # (fill)
-        "k_csr_count": 8 * f.nbentry + (64 + 4) * 5 * cells,
-        "k_csr_fill": 8 * f.nbentry + 12 * nnz + (64 + 4) * 5 * cells,
+    # bytes each kernel moves by design (face blocks are written once, read by
+    # the two cells beside each face)
+    blocks = 8 * 24 * 25 * (nfi + nfj)
+    by_design = {
+        "k_iface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * nfi,
+        "k_jface_blocks": JVP_SHARED_B_PER_CELL * cells + 8 * 24 * 25 * nfj,
+        "k_jac_csr_refresh": blocks + 8 * nnz + 64 * 5 * cells,
+        "ghost_tangents": 0,
+        "k_assemble_ghosts": 0,
     }
-    dom = max(alg_coo, key=lambda k: kernels[k]["total_ms"])
+    dom = max(names, key=lambda k: kernels[k]["total_ms"])
     avg_dom = kernels[dom]["avg_ms"]
     ach_dom = step_alg / (avg_dom * 1e-3) / 1e9
-    ach_dom_coo = alg_coo[dom] / (avg_dom * 1e-3) / 1e9
-    avg64 = face64_ms / max(face64_n, 1)
+    ach_dom_design = by_design[dom] / (avg_dom * 1e-3) / 1e9
     ach_res = RESIDUAL_B_PER_CELL * cells / (res_ms * 1e-3) / 1e9
-    asm_ms = sum(v["total_ms"] for v in kernels.values()) / args.steps
-    ach_asm = (8 * nnz + JVP_SHARED_B_PER_CELL * cells) / (asm_ms * 1e-3) / 1e9
+    ach_step = step_alg / (ms_step * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -308,16 +366,15 @@ def main():
         "config": {
             "workload": f"cylinder O-mesh {case.im}x{case.jm} (M=0.1, Re_D=50, seeded free stream): "
                         f"BCs + residual + full colored Jacobian ({nseeds} seeds"
-                        f"{', two i-zones' if nzones == 2 else ''}) into the reference COO, "
-                        "then pruned CSR",
+                        f"{', two i-zones' if nzones == 2 else ''}) in the pruned CSR, as a Newton "
+                        "iteration after the first (kept pattern checked per row, values rewritten)",
             "cells": cells,
             "seeds": nseeds,
-            "jacobian_entries": f.nbentry,
             "csr_nnz": nnz,
             "launch_shapes": tuned,
         },
         "roofline": {
-            "kernel": dom,
+            "kernel": dom + (" (k_jac_csr_rows<2,2,false> + <2,2,true>)" if dom == "k_jac_csr_refresh" else ""),
             "bound": "hbm",
             "achieved": ach_dom,
             "peak": HBM_PEAK_GBS,
@@ -328,34 +385,34 @@ def main():
             "launches": kernels[dom]["launches"],
             "algorithmic_bytes_per_launch": step_alg,
             "algorithmic_bytes": "8 B per CSR value + 96 B per cell (w and metrics) of the step",
-            "coo_inclusive": {"bytes_per_launch": alg_coo[dom], "achieved_gbs": ach_dom_coo,
-                              "frac": ach_dom_coo / HBM_PEAK_GBS,
-                              "note": "counts the reference-layout COO read and the CSR indices written"},
+            "by_design": {"bytes_per_launch": by_design[dom], "achieved_gbs": ach_dom_design,
+                          "frac": ach_dom_design / HBM_PEAK_GBS,
+                          "note": "the face blocks read, the CSR values and kept masks written"},
         },
-        "jacobian_assembly": {
-            "kernels": kernels,
-            "ms_per_assembly_in_timed_kernels": asm_ms,
-            "achieved_gbs_8B_per_csr_value_plus_96B_per_cell": ach_asm,
-            "frac": ach_asm / HBM_PEAK_GBS,
+        "step_kernels": kernels,
+        "step_achieved_gbs": ach_step,
+        "coo_route": {
+            "ms_per_step": coo_ms,
+            "kernels": coo_kernels,
+            "note": "the reference's route: colored assembly into its COO layout (16 B per slot, "
+                    f"{16 * f.nbentry / 1e9:.2f} GB), then the device prune + CSR; same CSR bit for bit",
         },
-        "graph_replay": {
+        "graph_replay_coo_route": {
             "ms_per_step": graph_ms,
-            "note": "BCs + residual + colored Jacobian captured once in a HIP graph and replayed, then the "
-                    "pruned CSR (its count reads nnz on the host, so it stays outside the graph)",
+            "note": "BCs + residual + colored Jacobian into the COO captured once in a HIP graph and replayed, "
+                    "then the pruned CSR",
         },
-        "fused_without_coo": {
-            "ms_per_step": fused_ms,
-            "kernels": fused_kernels,
-            "coo_bytes_not_allocated": 16 * f.nbentry,
-            "note": "same step straight into the CSR (bca_jacobian_colored_csr): same CSR bit for bit, "
-                    "no COO in memory; the slot values are formed twice (count, fill)",
+        "first_iteration_build": {
+            "ms_per_step": build_ms,
+            "kernels": build_kernels,
+            "note": "the full CSR build without a previous pattern: count pass, scan, fill pass",
         },
         "residual": {
             "ms": res_ms,
             "mcells_per_s": cells / (res_ms * 1e-3) / 1e6,
             "achieved_gbs": ach_res,
             "frac": ach_res / HBM_PEAK_GBS,
-            "face_kernel_avg_ms": avg64,
+            "face_kernel_avg_ms": face64["avg_ms"],
         },
         "jvp_seed_mcells_per_s": nseeds * cells / (ms_step * 1e-3) / 1e6,
     }
@@ -366,8 +423,8 @@ def main():
         out["learned_newton"] = learned_figure()
         out["residual_jvp_bl2048x1024"] = residual_jvp_figure()
         out["vjp_bl1024x256"] = vjp_figure()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_budget)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 
