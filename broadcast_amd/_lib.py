@@ -90,6 +90,9 @@ _SIGS = {
                      C.c_int, C.c_char_p, C.c_int],
     "bca_gemm_bf16": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp,
                       C.c_long, C.c_long, C.c_int, C.c_int],
+    "bca_lu_np": [_dp, _dp, C.c_int, C.c_long, C.c_double, _dp, _dp],
+    "bca_gemm_f64_sub": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp, C.c_long],
+    "bca_csr_spmv": [_dp, C.c_long, _dp, _dp, _dp, _dp, _dp],
 }
 
 _LONG_RESULT = ("bca_ctx_scratch_bytes", "bca_residual_noalloc_bytes")

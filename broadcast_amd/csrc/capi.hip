@@ -23,6 +23,7 @@
 #include "bcadj.h"
 #include "jaccsr.h"
 #include "gemm.h"
+#include "bandlu.h"
 
 using namespace bca;
 
@@ -1527,6 +1528,52 @@ int bca_jacobian_colored_csr_refresh(bca_ctx* c, const double* w, bca_metrics g,
   if (!rebuilt) return fail(-1, "null argument");
   return jac_csr(c, w, g, p, ops, nops, coefdiag, nzones, zparam, mini, indptr, indices, data, cap, nnz, true,
                  rebuilt);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ Newton linear solve
+extern "C" {
+
+int bca_lu_np(void* stream, double* A, int n, long ld, double tiny, int* bad, double* work) {
+  if (!A || !bad || !work) return fail(-1, "null argument");
+  if (n < 1 || ld < n) return fail(-1, "bca_lu_np: need n >= 1 and ld >= n");
+  const hipStream_t s = (hipStream_t)stream;
+  for (int k0 = 0; k0 < n; k0 += LU_NB) {
+    const int nb = std::min(LU_NB, n - k0), m = n - k0 - nb;
+    double* T = A + (long)k0 * ld + k0;
+    const int nrb = (m + LU_THREADS - 1) / LU_THREADS;
+    k_lu_panel_np<<<1 + 2 * nrb, LU_THREADS, 0, s>>>(T, ld, nb, m, nrb, tiny, bad, work);
+    BCA_LAUNCHED();
+    if (m == 0) break;
+    const dim3 g((m + 63) / 64, (m + 63) / 64);
+    k_gemm_f64_sub<<<g, 256, 0, s>>>(m, m, nb, T + (long)nb * ld, ld, T + nb, ld, T + (long)nb * ld + nb, ld, T, ld,
+                                     work, nb);
+    BCA_LAUNCHED();
+  }
+  return 0;
+}
+
+int bca_gemm_f64_sub(void* stream, int M, int N, int K, const double* A, long lda, const double* B, long ldb,
+                     double* C, long ldc) {
+  if (!A || !B || !C) return fail(-1, "null argument");
+  if (M < 0 || N < 0 || K < 0) return fail(-1, "negative size");
+  const hipStream_t s = (hipStream_t)stream;
+  for (int k0 = 0; k0 < K; k0 += LU_NB) {
+    const int kc = std::min(LU_NB, K - k0);
+    const dim3 g((N + 63) / 64, (M + 63) / 64);
+    k_gemm_f64_sub<<<g, 256, 0, s>>>(M, N, kc, A + k0, lda, B + (long)k0 * ldb, ldb, C, ldc);
+    BCA_LAUNCHED();
+  }
+  return 0;
+}
+
+int bca_csr_spmv(void* stream, long n, const int* indptr, const int* indices, const double* data, const double* x,
+                 double* y) {
+  if (!indptr || !indices || !data || !x || !y) return fail(-1, "null argument");
+  k_csr_spmv<<<grid1(n, 256), 256, 0, (hipStream_t)stream>>>(n, indptr, indices, data, x, y);
+  BCA_LAUNCHED();
+  return 0;
 }
 
 }  // extern "C"

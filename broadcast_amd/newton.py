@@ -44,10 +44,26 @@ from . import device as dev
 
 class BlockBandLU:
     """Dense block-banded LU on the device of an n x n CSR matrix whose
-    unknown u = e + 5 (j + jm i) (0-based) is permuted to outer-major blocks."""
+    unknown u = e + 5 (j + jm i) (0-based) is permuted to outer-major blocks.
 
-    def __init__(self, im, jm, outer="i", kb=None, group=1):
+    The band half-width kb (in blocks) is the widest coupling of every block
+    row but the first: on the O-mesh (j-line blocks) the wall row j = 1 also
+    reaches j = 5 (the off-centred Euler flux of the face j = 2), one block
+    beyond the radial stencil's 3.  Such far blocks of block row 0 are kept
+    outside the band (eliminating block 0 turns them into in-band updates of
+    block rows 1..kb), so the band stays 2 kb + 1 blocks wide: with kb = 3
+    instead of 4, 9 instead of 16 Schur products per block step.
+
+    With look-ahead (device tensors), block step I updates the next diagonal
+    block first and factors it on a side stream while the remaining Schur
+    products of step I run on the current stream."""
+
+    def __init__(self, im, jm, outer="i", kb=None, group=1, lookahead=True, pivot=None):
         self.t = dev.torch()
+        # pivot=None: the project's LU without pivoting on the device (bca_lu_np,
+        # FP64 matrix cores) for the diagonal blocks; torch's pivoted LU on the
+        # CPU or when asked (pivot=True) or when a pivot comes out too small
+        self.pivot = pivot
         self.im, self.jm = im, jm
         self.outer = outer
         self.n = 5 * im * jm
@@ -57,6 +73,7 @@ class BlockBandLU:
         self.nb = -(-lines // group)
         self.np = self.nb * self.B                 # padded with identity rows past n
         self.kb = kb
+        self.lookahead = lookahead
 
     def _perm(self, u):
         """Block-major index of the reference's unknown u."""
@@ -75,35 +92,169 @@ class BlockBandLU:
         rows = self._perm(t.repeat_interleave(t.arange(n, device=dv), counts))
         cols = self._perm(indices.long())
         K, J = rows // B, cols // B
-        kb = int((J - K).abs().max()) if self.kb is None else self.kb
-        self.kb = kb
+        d = J - K
+        if self.kb is None:
+            rest = K > 0
+            kb = int(d[rest].abs().max()) if bool(rest.any()) else 0
+            kb = max(kb, int((-d).max()), 1 if self.nb > 1 else 0)
+            self.kb = min(kb, self.nb - 1)
+        kb = self.kb
         W = (2 * kb + 1) * B
-        d = J - K + kb
-        if bool(((d < 0) | (d > 2 * kb)).any()):
+        far = d > kb
+        if bool((far & (K > 0)).any()) or bool((d < -kb).any()):
             raise ValueError("entry outside the block band")
+        self.nfar = int(d[far].max()) - kb if bool(far.any()) else 0
+        inb = ~far
         R = t.zeros((self.nb, B, W), dtype=data.dtype, device=dv)
-        R.view(-1).index_put_(((K * B + rows % B) * W + d * B + cols % B,), data)
+        R.view(-1).index_put_((((K * B + rows % B) * W + (d + kb) * B + cols % B)[inb],), data[inb])
+        # far blocks of block row 0: columns kb+1 .. kb+nfar
+        F = t.zeros((B, self.nfar * B), dtype=data.dtype, device=dv)
+        if self.nfar:
+            F.view(-1).index_put_((((rows % B) * (self.nfar * B) + (d - kb - 1) * B + cols % B)[far],), data[far])
         if self.np > n:
             pad = t.arange(n, self.np, device=dv)
             R.view(-1)[((pad // B) * B + pad % B) * W + kb * B + pad % B] = 1.0
-        self.R = R
-        self.lu, self.piv = [], []
+        self.R, self.F = R, F
+        self.lu, self.piv = [None] * self.nb, [None] * self.nb
         # block Gaussian elimination: D_I = A(I, I) after the updates of the
         # steps before; X(I, J) = D_I^-1 A(I, J) stored in place of A(I, J);
         # A(K, J) -= A(K, I) X(I, J) for K, J in I+1..I+kb
+        self.np_lu = self.pivot is not True and dv.type == "cuda"
+        if self.np_lu:
+            self._bad = t.zeros(1, dtype=t.int32, device=dv)
+            if self.np_factor(R, F):
+                return self
+            # a pivot came out too small: refactor with partial pivoting
+            self.np_lu = False
+            self.pivot, self.kb = True, None
+            return self.factor(indptr, indices, data)
+        side = t.cuda.Stream() if (self.lookahead and dv.type == "cuda") else None
+        cur = t.cuda.current_stream() if side is not None else None
+        ready = None
         for I in range(self.nb):
-            LU, piv, _ = t.linalg.lu_factor_ex(R[I, :, kb * B:(kb + 1) * B])
-            self.lu.append(LU)
-            self.piv.append(piv)
+            if ready is not None:
+                cur.wait_event(ready)
+                ready = None
+            if self.lu[I] is None:
+                self.lu[I], self.piv[I], _ = t.linalg.lu_factor_ex(R[I, :, kb * B:(kb + 1) * B])
+            LU, piv = self.lu[I], self.piv[I]
             hi = min(I + kb, self.nb - 1)
             if hi == I:
                 continue
             Xr = R[I, :, (kb + 1) * B:(kb + 1 + hi - I) * B]
             Xr.copy_(t.linalg.lu_solve(LU, piv, Xr))
+            if I == 0 and self.nfar:
+                F.copy_(t.linalg.lu_solve(LU, piv, F))
+            # the next diagonal block first, then its factorization beside
+            # the rest of this step's updates
+            R[I + 1, :, kb * B:(kb + 1) * B].sub_(R[I + 1, :, (kb - 1) * B:kb * B] @ Xr[:, :B])
+            if side is not None:
+                ev = t.cuda.Event()
+                ev.record(cur)
+                side.wait_event(ev)
+                with t.cuda.stream(side):
+                    A1 = R[I + 1, :, kb * B:(kb + 1) * B]
+                    self.lu[I + 1], self.piv[I + 1], _ = t.linalg.lu_factor_ex(A1)
+                    ready = t.cuda.Event()
+                    ready.record(side)
+                self.lu[I + 1].record_stream(cur)
+                self.piv[I + 1].record_stream(cur)
             for dd in range(1, hi - I + 1):
                 K2 = I + dd
-                R[K2, :, (kb - dd + 1) * B:(kb - dd + 1 + hi - I) * B].sub_(R[K2, :, (kb - dd) * B:(kb - dd + 1) * B] @ Xr)
+                L = R[K2, :, (kb - dd) * B:(kb - dd + 1) * B]
+                if dd == 1:
+                    # the diagonal block of K2 = I+1 is done: the blocks right of it
+                    if hi - I > 1:
+                        R[K2, :, (kb + 1) * B:(kb + hi - I) * B].sub_(L @ Xr[:, B:])
+                else:
+                    R[K2, :, (kb - dd + 1) * B:(kb - dd + 1 + hi - I) * B].sub_(L @ Xr)
+                if I == 0 and self.nfar:
+                    # far block f (absolute block kb + f) lies at offset kb + f - K2 of row K2
+                    for f in range(1, self.nfar + 1):
+                        off = kb + f - K2
+                        if off <= kb:
+                            R[K2, :, (kb + off) * B:(kb + off + 1) * B].sub_(L @ F[:, (f - 1) * B:f * B])
+        if ready is not None:
+            cur.wait_event(ready)
         return self
+
+    def np_factor(self, R, F):
+        """Block elimination with the project's unpivoted LU of each diagonal
+        block (bca_lu_np) and triangular solves / Schur products by BLAS;
+        the next diagonal block is factored on a side stream while the rest
+        of the step's Schur products run.  False if a pivot was too small."""
+        import ctypes as C
+        from . import _lib
+
+        t = self.t
+        B, kb = self.B, self.kb
+        L_ = _lib.lib()
+        W = R.shape[2]
+        cur = t.cuda.current_stream()
+        side = t.cuda.Stream() if self.lookahead else cur
+        self._bad.zero_()
+        # one 32 x 32 scratch tile per stream the factorizations run on
+        work = [t.empty(32 * 32, dtype=t.float64, device=R.device) for _ in range(2)]
+
+        def lu_np(I, stream):
+            D = R[I, :, kb * B:(kb + 1) * B]
+            _lib.check(L_.bca_lu_np(C.c_void_p(stream.cuda_stream), C.c_void_p(D.data_ptr()), B, W, 1e-14,
+                                    C.c_void_p(self._bad.data_ptr()), C.c_void_p(work[stream is side].data_ptr())),
+                       "bca_lu_np")
+
+        lu_np(0, cur)
+        ready = None
+        for I in range(self.nb):
+            if ready is not None:
+                cur.wait_event(ready)
+                ready = None
+            hi = min(I + kb, self.nb - 1)
+            if hi == I:
+                continue
+            D = R[I, :, kb * B:(kb + 1) * B]
+            Xr = R[I, :, (kb + 1) * B:(kb + 1 + hi - I) * B]
+            Xr.copy_(t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, Xr, upper=False, unitriangular=True),
+                                               upper=True))
+            if I == 0 and self.nfar:
+                F.copy_(t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, F, upper=False, unitriangular=True),
+                                                  upper=True))
+            R[I + 1, :, kb * B:(kb + 1) * B].sub_(R[I + 1, :, (kb - 1) * B:kb * B] @ Xr[:, :B])
+            if self.lookahead:
+                ev = t.cuda.Event()
+                ev.record(cur)
+                side.wait_event(ev)
+                lu_np(I + 1, side)
+                ready = t.cuda.Event()
+                ready.record(side)
+            else:
+                lu_np(I + 1, cur)
+            for dd in range(1, hi - I + 1):
+                K2 = I + dd
+                L = R[K2, :, (kb - dd) * B:(kb - dd + 1) * B]
+                if dd == 1:
+                    if hi - I > 1:
+                        R[K2, :, (kb + 1) * B:(kb + hi - I) * B].sub_(L @ Xr[:, B:])
+                else:
+                    R[K2, :, (kb - dd + 1) * B:(kb - dd + 1 + hi - I) * B].sub_(L @ Xr)
+                if I == 0 and self.nfar:
+                    for f in range(1, self.nfar + 1):
+                        off = kb + f - K2
+                        if off <= kb:
+                            R[K2, :, (kb + off) * B:(kb + off + 1) * B].sub_(L @ F[:, (f - 1) * B:f * B])
+        if ready is not None:
+            cur.wait_event(ready)
+        return int(self._bad.item()) == 0
+
+    def _solve_diag(self, I, v):
+        """D_I^-1 v for one block (v: (B,) or (B, k))."""
+        t = self.t
+        if not self.np_lu:
+            return t.linalg.lu_solve(self.lu[I], self.piv[I], v.unsqueeze(1) if v.dim() == 1 else v).reshape(v.shape)
+        kb, B = self.kb, self.B
+        D = self.R[I, :, kb * B:(kb + 1) * B]
+        m = v.unsqueeze(1) if v.dim() == 1 else v
+        y = t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, m, upper=False, unitriangular=True), upper=True)
+        return y.reshape(v.shape)
 
     def solve(self, b):
         """x with A x = b, b in the reference's unknown order."""
@@ -116,16 +267,57 @@ class BlockBandLU:
         y = t.cat((y, y.new_zeros(self.np - self.n))).reshape(nb, B)
         R = self.R
         for I in range(nb):
-            y[I] = t.linalg.lu_solve(self.lu[I], self.piv[I], y[I].unsqueeze(1)).squeeze(1)
+            y[I] = self._solve_diag(I, y[I])
             for dd in range(1, min(kb, nb - 1 - I) + 1):
                 y[I + dd] -= R[I + dd, :, (kb - dd) * B:(kb - dd + 1) * B] @ y[I]
         for I in range(nb - 1, -1, -1):
             for dd in range(1, min(kb, nb - 1 - I) + 1):
                 y[I] -= R[I, :, (kb + dd) * B:(kb + dd + 1) * B] @ y[I + dd]
+            if I == 0 and self.nfar:
+                for f in range(1, self.nfar + 1):
+                    if kb + f < nb:
+                        y[0] -= self.F[:, (f - 1) * B:f * B] @ y[kb + f]
         y = y.reshape(-1)[:self.n]
         if self.outer == "i":
             return y
         return y.reshape(self.jm, self.im, 5).permute(1, 0, 2).reshape(-1)
+
+
+def csr_residual(csr, x, b):
+    """b - A x for the CSR (indptr, indices, data) on the device."""
+    import ctypes as C
+    from . import _lib
+
+    t = dev.torch()
+    indptr, indices, data = csr
+    y = t.empty_like(b)
+    _lib.check(_lib.lib().bca_csr_spmv(dev.current_stream_ptr(), b.numel(), C.c_void_p(indptr.data_ptr()),
+                                       C.c_void_p(indices.data_ptr()), C.c_void_p(data.data_ptr()),
+                                       C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "bca_csr_spmv")
+    return b - y
+
+
+def refine(lu, csr, b, x, rtol=1e-12, steps=3):
+    """Iterative refinement of x (in place) for A x = b with the factors of
+    lu, until ||b - A x|| <= rtol ||b||, a correction no longer halves the
+    residual (its rounding floor), or `steps` corrections: the guard of the
+    unpivoted factorization.  Returns (corrections, final relative
+    residual)."""
+    bn = float(b.norm())
+    k = 0
+    r = csr_residual(csr, x, b)
+    rel = float(r.norm()) / max(bn, 1e-300)
+    while rel > rtol and k < steps:
+        dx = lu.solve(r)
+        x += dx
+        k += 1
+        r = csr_residual(csr, x, b)
+        new = float(r.norm()) / max(bn, 1e-300)
+        if new > 0.5 * rel:
+            rel = new
+            break
+        rel = new
+    return k, rel
 
 
 @dataclass
@@ -133,6 +325,8 @@ class NewtonLog:
     norms: list = field(default_factory=list)      # max_e ||R_e||_2 before each update
     relax: list = field(default_factory=list)      # r of each iteration
     seconds: list = field(default_factory=list)    # wall time of each iteration
+    split: list = field(default_factory=list)      # per update: (jacobian, factor, solve) seconds
+    refinements: list = field(default_factory=list)  # per update: (refinement steps, final relative residual)
     rollbacks: int = 0
     converged: bool = False
 
@@ -160,10 +354,8 @@ def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth
         dy = float(yc[4, 3] - yc[3, 3])
     vol_int = flow.vol.view(flow.nj, flow.ni)[3:-3, 3:-3]      # (jm, im)
     res = flow.new_state()
-    bufs = flow.jacobian_buffers()
     outer = "j" if case.kind == "cyl" else "i"
     norm0 = ninf0 = None
-    first = True
     w_prev, l2_prev = None, None
     it = 0
     while it < ite:
@@ -197,15 +389,70 @@ def newton(flow, w, ite=10, cfl=1e10, tol=1e-10, log=None, verbose=False, growth
         log.relax.append(r)
         dtm1 = (1.0 / mach + 1.0) / (cfl * dy)
         coefdiag = (r * dtm1 * vol_int).contiguous().reshape(-1)   # (im, jm) in Fortran order
-        jac, ia, ja = flow.jacobian_colored(w, coefdiag=coefdiag, bufs=bufs, write_index=first)
-        first = False
-        indptr, indices, data = flow.csr(jac)
+        # the colored Jacobian straight into the pruned CSR; from the second
+        # iteration on only the values are rewritten while every row keeps
+        # its kept set (the same matrix as jacobian_colored + csr)
+        indptr, indices, data = flow.jacobian_csr(w, coefdiag=coefdiag, refresh=True)
+        t.cuda.synchronize()
+        t1 = time.perf_counter()
         lu = BlockBandLU(flow.im, flow.jm, outer=outer, group=group).factor(indptr, indices, data)
-        dw = lu.solve(interior(flow, res).reshape(-1))
+        t.cuda.synchronize()
+        t2 = time.perf_counter()
+        rhs = interior(flow, res).reshape(-1)
+        dw = lu.solve(rhs)
+        log.refinements.append(refine(lu, (indptr, indices, data), rhs, dw))
+        del lu
         w_prev, l2_prev = w.clone(), l2
         wi = w.view(5, flow.nj, flow.ni)[:, 3:-3, 3:-3]
         wi += dw.reshape(flow.im, flow.jm, 5).permute(2, 1, 0)
         t.cuda.synchronize()
-        log.seconds.append(time.perf_counter() - t0)
+        t3 = time.perf_counter()
+        log.split.append((t1 - t0, t2 - t1, t3 - t2))
+        log.seconds.append(t3 - t0)
     log.cfl = cfl
     return log
+
+
+def continuation(flow_at, target, start, w=None, steps=4, ite=40, cfl=1e21, tol=1e-10, min_step=None, log=None,
+                 verbose=False):
+    """Continuation Newton in one flow parameter (the Reynolds number here):
+    Newton at `start` (from w, or the free stream of flow_at(start)), then at
+    parameters stepping to `target`, each solve starting from the previous
+    baseflow; a step that does not converge is halved (down to min_step).
+    flow_at(p) gives the Flow at parameter p on the same mesh.  Returns
+    (flow at target, w at target, list of (p, NewtonLog)).  No reference
+    counterpart: the reference's drivers start every solve from their own
+    initial state."""
+    from .flow import to_device
+
+    t = dev.torch()
+    f = flow_at(start)
+    w = to_device(f.case.w) if w is None else w
+    hist = []
+    lg = newton(f, w, ite=ite, cfl=cfl, tol=tol)
+    hist.append((start, lg))
+    if not lg.converged:
+        raise RuntimeError(f"continuation: no converged start at {start}")
+    p = start
+    dp = (target - start) / max(steps, 1)
+    min_step = abs(dp) / 16 if min_step is None else min_step
+    while (target - p) * np.sign(dp) > 1e-12:
+        q = p + dp
+        if (target - q) * np.sign(dp) < 1e-9 * abs(dp):
+            q = target
+        fq = flow_at(q)
+        wq = w.clone()
+        lq = newton(fq, wq, ite=ite, cfl=cfl, tol=tol)
+        hist.append((q, lq))
+        if verbose:
+            print(f"continuation {p:.3g} -> {q:.3g}: {len(lq.norms) - 1} Newton steps, converged {lq.converged}",
+                  flush=True)
+        if lq.converged:
+            p, w, f = q, wq, fq
+        else:
+            dp /= 2.0
+            if abs(dp) < min_step:
+                raise RuntimeError(f"continuation stalled at {p} (step {dp})")
+        del wq
+    t.cuda.synchronize()
+    return f, w, hist

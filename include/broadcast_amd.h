@@ -312,6 +312,24 @@ int bca_jacobian_colored_csr_refresh(bca_ctx *ctx, const double *w, bca_metrics 
                                      int zparam, double mini, int *indptr, int *indices, double *data, long cap,
                                      long *nnz, int *rebuilt);
 
+/* ---- Newton linear solve ---------------------------------------------------- */
+/* The reference solves each Newton system with MUMPS through PETSc
+ * (misc/PETSc_func.py:137-152,247-263; BROADCAST_npz.py:1152-1157).  The
+ * project's driver factors the block-banded matrix by block Gaussian
+ * elimination; these entries are its dense parts.
+ * bca_lu_np: in-place LU without pivoting (L unit lower, U upper) of the
+ * n x n row-major block at A (leading dimension ld): 32-column panels, the
+ * trailing updates on the FP64 matrix cores.  *bad (device int, set by the
+ * caller to 0) becomes 1 if a pivot falls below tiny times its tile's
+ * largest entry; work: device scratch of 32 x 32 doubles.
+ * bca_gemm_f64_sub: C -= A B, row-major, FP64 matrix cores.
+ * bca_csr_spmv: y = A x for a CSR matrix (the residual of a refined solve). */
+int bca_lu_np(void *hip_stream, double *A, int n, long ld, double tiny, int *bad, double *work);
+int bca_gemm_f64_sub(void *hip_stream, int M, int N, int K, const double *A, long lda, const double *B,
+                     long ldb, double *C, long ldc);
+int bca_csr_spmv(void *hip_stream, long n, const int *indptr, const int *indices, const double *data,
+                 const double *x, double *y);
+
 /* ---- BF16 GEMM of the learned Newton guess ---------------------------------- */
 /* No reference counterpart (the reference has no learned part).  C (fp32,
  * M x N) = A (bf16 bit patterns, M x K) * B (bf16, K x N), FP32 accumulation on

@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 from oracle import driver as D
-from broadcast_amd.cases import bl_case
+from broadcast_amd.cases import bl_case, cylinder_case
 
 pytestmark = pytest.mark.gpu
 
@@ -113,3 +113,121 @@ def test_offedge_boundary_box_is_refused(torch_mod):
                                   C.c_void_p(jac.data_ptr()), C.c_void_p(ia.data_ptr()), C.c_void_p(ja.data_ptr()),
                                   f.nbentry, 1)
     assert rc == -1
+
+
+def test_reynolds_continuation_reaches_the_baseflow(torch_mod):
+    """Continuation Newton in Re on the O-mesh (the learned guess's fallback):
+    from a converged baseflow at Re 40 in steps to Re 150; the end state is
+    a baseflow of the target Reynolds number (||R|| < 1e-10 by the oracle's
+    residual of that state) and equals the one plain Newton reaches from the
+    free stream when that converges."""
+    from broadcast_amd import newton as N
+    from broadcast_amd.flow import Flow, to_device, to_host
+
+    im, jm, mach = 48, 32, 0.2
+
+    def flow_at(re):
+        return Flow(cylinder_case(im, jm, mach=mach, reynolds=float(re), noise=0.0))
+
+    f, w, hist = N.continuation(flow_at, 150.0, 40.0, steps=3)
+    assert [p for p, lg in hist if lg.converged][-1] == 150.0
+    assert all(lg.converged for p, lg in hist[1:] if p == 150.0)
+    O = oracle.Oracle()
+    case = f.case
+    g = D.geometry(O, case)
+    wh = np.array(to_host(w, f.shape_w), order="F", copy=True)
+    D.apply_bcs(O, case, g, wh)
+    norm, _ = O.compute_norml2inf(D.residual(O, case, g, wh), case.im, case.jm, case.gh)
+    assert np.max(norm) < 1e-10 * 1.05, norm
+    fp = flow_at(150.0)
+    wp = to_device(fp.case.w)
+    lg = N.newton(fp, wp, ite=40, cfl=1e21, tol=1e-10)
+    if lg.converged:
+        a = w.view(5, f.nj, f.ni)[:, 3:-3, 3:-3]
+        b = wp.view(5, f.nj, f.ni)[:, 3:-3, 3:-3]
+        assert float((a - b).norm() / b.norm()) < 1e-8
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 130, 45), (257, 33, 100), (1, 5, 3)])
+def test_gemm_f64_matches_torch(torch_mod, M, N, K):
+    """bca_gemm_f64_sub (C -= A B on the FP64 matrix cores, row-major,
+    ragged sizes, K in chunks of the panel width) against torch's FP64
+    product: 1e-14 of the product's scale."""
+    import ctypes as C
+    from broadcast_amd import _lib
+
+    t = torch_mod
+    g = t.Generator(device="cuda").manual_seed(M * 1000 + N * 10 + K)
+    A = t.randn(M, K + 3, dtype=t.float64, device="cuda", generator=g)[:, :K]   # strided rows
+    B = t.randn(K, N, dtype=t.float64, device="cuda", generator=g)
+    Cm = t.randn(M, N + 5, dtype=t.float64, device="cuda", generator=g)
+    ref = Cm[:, :N] - A @ B
+    s = t.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib().bca_gemm_f64_sub(C.c_void_p(s), M, N, K, C.c_void_p(A.data_ptr()), A.stride(0),
+                                           C.c_void_p(B.data_ptr()), B.stride(0), C.c_void_p(Cm.data_ptr()),
+                                           Cm.stride(0)), "bca_gemm_f64_sub")
+    scale = float((A.abs() @ B.abs()).max()) + 1.0
+    assert float((Cm[:, :N] - ref).abs().max()) <= 1e-14 * scale
+
+
+@pytest.mark.parametrize("n", [37, 200, 1001])
+def test_lu_np_reproduces_the_block(torch_mod, n):
+    """bca_lu_np on a block of the Newton matrices' kind (diagonally heavy,
+    row-major inside a wider array): L U reproduces it to 1e-13 normwise,
+    no small pivot is reported; a block with a zero leading pivot is."""
+    import ctypes as C
+    from broadcast_amd import _lib
+
+    t = torch_mod
+    g = t.Generator(device="cuda").manual_seed(n)
+    A0 = t.randn(n, n, dtype=t.float64, device="cuda", generator=g) + 4 * n ** 0.5 * t.eye(n, dtype=t.float64,
+                                                                                     device="cuda")
+    big = t.zeros(n, n + 17, dtype=t.float64, device="cuda")
+    big[:, 7:7 + n] = A0
+    bad = t.zeros(1, dtype=t.int32, device="cuda")
+    work = t.empty(32 * 32, dtype=t.float64, device="cuda")
+    s = t.cuda.current_stream().cuda_stream
+    Av = big[:, 7:7 + n]
+    _lib.check(_lib.lib().bca_lu_np(C.c_void_p(s), C.c_void_p(Av.data_ptr()), n, big.stride(0), 1e-14,
+                                    C.c_void_p(bad.data_ptr()), C.c_void_p(work.data_ptr())), "bca_lu_np")
+    L = t.tril(Av, -1) + t.eye(n, dtype=t.float64, device="cuda")
+    U = t.triu(Av)
+    assert int(bad.item()) == 0
+    assert float((L @ U - A0).norm() / A0.norm()) < 1e-13
+    assert t.equal(big[:, :7], t.zeros_like(big[:, :7])) and t.equal(big[:, 7 + n:], t.zeros_like(big[:, 7 + n:]))
+    Z = A0.clone()
+    Z[0, 0] = 0.0
+    _lib.check(_lib.lib().bca_lu_np(C.c_void_p(s), C.c_void_p(Z.data_ptr()), n, n, 1e-14,
+                                    C.c_void_p(bad.data_ptr()), C.c_void_p(work.data_ptr())), "bca_lu_np")
+    assert int(bad.item()) == 1
+
+
+@pytest.mark.parametrize("mk,outer", [(lambda: bl_case(40, 24, seed=71), "i"),
+                                      (lambda: cylinder_case(48, 32, seed=72), "j")])
+def test_block_band_lu_on_device_matches_dense_solve(torch_mod, mk, outer):
+    """The Newton driver's solve on the device (unpivoted project LU of the
+    diagonal blocks, far blocks outside the band, look-ahead) on the colored
+    Jacobian from the GPU path, refined with the CSR residual: the same
+    solution as torch's pivoted dense solve to 1e-10, residual <= 1e-13."""
+    from broadcast_amd import newton as N
+    from broadcast_amd.flow import Flow, to_device
+
+    t = torch_mod
+    case = mk()
+    f = Flow(case)
+    w = to_device(case.w)
+    f.apply_bcs(w)
+    cd = t.full((case.im * case.jm,), 1e-3, dtype=t.float64, device="cuda")
+    indptr, indices, data = f.jacobian_csr(w, coefdiag=cd)
+    n = 5 * case.im * case.jm
+    A = t.sparse_csr_tensor(indptr.long(), indices.long(), data, size=(n, n)).to_dense()
+    b = t.randn(n, dtype=t.float64, device="cuda", generator=t.Generator(device="cuda").manual_seed(3))
+    ref = t.linalg.solve(A, b)
+    lu = N.BlockBandLU(case.im, case.jm, outer=outer).factor(indptr, indices, data)
+    assert lu.np_lu and lu.kb == 3
+    x = lu.solve(b)
+    k, rel = N.refine(lu, (indptr, indices, data), b, x)
+    anorm = float(A.abs().sum(dim=1).max())
+    berr = float((b - A @ x).abs().max()) / (anorm * float(x.abs().max()) + float(b.abs().max()))
+    assert rel <= 1e-12 and berr < 1e-15, (k, rel, berr)
+    assert float((x - ref).norm() / ref.norm()) < 1e-10

@@ -45,4 +45,4 @@ def test_block_band_lu_matches_sparse_solve(O, mk, outer, group):
     berr_ref = np.linalg.norm(J @ ref - b, np.inf) / (anorm * np.linalg.norm(ref, np.inf) + np.linalg.norm(b, np.inf))
     assert berr <= max(10 * berr_ref, 1e-15), (berr, berr_ref)
     assert np.linalg.norm(x - ref) <= 1e-8 * np.linalg.norm(ref)
-    assert lu.kb <= -(-(4 if outer == "j" else 3) // group)
+    assert lu.kb <= -(-3 // group)
