@@ -84,7 +84,8 @@ class BlockBandLU:
         i = u // (5 * self.jm)
         return e + 5 * (i + self.im * j)
 
-    def factor(self, indptr, indices, data):
+    def factor(self, indptr, indices, data, shift=None):
+        """Factor the CSR matrix (minus shift * I when shift is given)."""
         t = self.t
         dv = data.device
         n, B = self.n, self.B
@@ -111,6 +112,9 @@ class BlockBandLU:
         F = t.zeros((B, self.nfar * B), dtype=data.dtype, device=dv)
         if self.nfar:
             F.view(-1).index_put_((((rows % B) * (self.nfar * B) + (d - kb - 1) * B + cols % B)[far],), data[far])
+        if shift is not None:
+            u = t.arange(n, device=dv)
+            R.view(-1)[((u // B) * B + u % B) * W + kb * B + u % B] -= shift
         if self.np > n:
             pad = t.arange(n, self.np, device=dv)
             R.view(-1)[((pad // B) * B + pad % B) * W + kb * B + pad % B] = 1.0
@@ -119,7 +123,7 @@ class BlockBandLU:
         # block Gaussian elimination: D_I = A(I, I) after the updates of the
         # steps before; X(I, J) = D_I^-1 A(I, J) stored in place of A(I, J);
         # A(K, J) -= A(K, I) X(I, J) for K, J in I+1..I+kb
-        self.np_lu = self.pivot is not True and dv.type == "cuda"
+        self.np_lu = self.pivot is not True and dv.type == "cuda" and data.dtype == t.float64
         if self.np_lu:
             self._bad = t.zeros(1, dtype=t.int32, device=dv)
             if self.np_factor(R, F):
@@ -127,7 +131,7 @@ class BlockBandLU:
             # a pivot came out too small: refactor with partial pivoting
             self.np_lu = False
             self.pivot, self.kb = True, None
-            return self.factor(indptr, indices, data)
+            return self.factor(indptr, indices, data, shift)
         side = t.cuda.Stream() if (self.lookahead and dv.type == "cuda") else None
         cur = t.cuda.current_stream() if side is not None else None
         ready = None
