@@ -192,12 +192,12 @@ def blasius_state(xc, yc, phys: Physics):
 
 def bl_case(im: int, jm: int, *, mach=0.5, reynolds=1e5, x_le=0.1, length=1.0,
             dy1=1e-4, ratio=1.03, noise=1e-3, seed=0, k2=1.01, k4=1.0,
-            name=None) -> Case:
+            name=None, gh=GH) -> Case:
     """Flat-plate boundary layer (BROADCAST_npz.py:474-799 set-up, own mesh and
     initial state).  Interior nodes: x uniform on [x_le, x_le+length], y
     geometric from 0 with first spacing dy1 and ratio.  State: Blasius-type +
-    N(0, (noise*q_inf)^2) on the interior (q_inf = (1,1,1,1,rhoE_inf))."""
-    gh = GH
+    N(0, (noise*q_inf)^2) on the interior (q_inf = (1,1,1,1,rhoE_inf)).
+    gh = 2 gives the ghost width of the 3rd-order scheme."""
     phys = Physics(mach=mach, reynolds=reynolds, k2=k2, k4=k4)
     ni, nj = im + 2 * gh, jm + 2 * gh
     xn = np.linspace(x_le, x_le + length, im + 1)
@@ -258,13 +258,13 @@ def uneven_bl_cases(n: int = 8, seed: int = 2000, im_range=(128, 1024), jm_range
 
 def cylinder_case(im: int, jm: int, *, mach=0.1, reynolds=50.0, r0=0.5,
                   r1=100.0, dr1=5e-3, noise=1e-4, seed=0, k2=0.0, k4=1.0,
-                  name=None) -> Case:
+                  name=None, gh=GH) -> Case:
     """O-mesh around a cylinder of diameter 1 (cylinder.py:431-530 set-up, own
     mesh): i runs clockwise in theta (so that (i, j) is right-handed like the
     flat-plate block), j runs outward from r0 to r1 with first spacing dr1 and
     a geometric ratio q solving dr1 (q^jm - 1)/(q - 1) = r1 - r0.  State:
-    uniform free stream along +x plus N(0, noise^2) on the interior."""
-    gh = GH
+    uniform free stream along +x plus N(0, noise^2) on the interior.
+    gh = 2 gives the ghost width of the 3rd-order scheme."""
     phys = Physics(mach=mach, reynolds=reynolds, k2=k2, k4=k4)
     ni, nj = im + 2 * gh, jm + 2 * gh
     span = r1 - r0

@@ -202,15 +202,23 @@ Phys to_phys(const bca_phys* p) {
 }
 Metric to_metric(const bca_metrics& g) { return Metric{g.nx, g.ny, g.vol, g.volf}; }
 
+// gh = 3 for the 5th-order scheme and every Jacobian path, gh = 2 for the
+// 3rd-order scheme alone (the reference's gh = (order + 1) / 2,
+// BROADCAST_npz.py:501-502)
 int check_mesh(const Mesh& m) {
-  if (m.gh != 3) return fail(-1, "only gh = 3 (5th-order DNC) is supported");
+  if (m.gh != 3 && m.gh != 2) return fail(-1, "gh must be 3 (5th-order DNC) or 2 (3rd-order DNC)");
   if (m.im < 5 || m.jm < 5) return fail(-1, "im and jm must be >= 5 (wall stencil reaches j = 5)");
+  return 0;
+}
+int need_gh3(const bca_ctx* c) {
+  if (c && c->m.gh != 3) return fail(-1, "the 5th-order scheme and its Jacobian paths need gh = 3");
   return 0;
 }
 
 // The residual pipeline on scalar type T; residu == nullptr stops after the faces.
 template <class T>
-int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g, const Phys& ph) {
+int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g, const Phys& ph,
+                 bool nowall = false, int ord = 5) {
   const Mesh& m = c->m;
   hipStream_t s = c->stream;
   const std::string sfx = std::string("<") + tname<T>() + ">";
@@ -222,15 +230,20 @@ int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g
   const dim3 b(64, 4);
   {
     Timed t(c, "k_grad" + sfx);
-    k_grad<T><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
+    if (ord == 3) k_grad<T, 3><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
+    else k_grad<T><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
   }
   BCA_LAUNCHED();
-  k_grad_ghost<T><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, wk.grad);
+  k_grad_ghost<T><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, wk.grad);
   BCA_LAUNCHED();
   const dim3 bf = sizeof(T) > 8 ? dim3(64, 1) : c->tune_face;
   {
     Timed t(c, "k_face" + sfx);
-    k_face<T><<<grid2(m.im + 1, m.jm + 1, bf), bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+    const dim3 gf = grid2(m.im + 1, m.jm + 1, bf);
+    if (ord == 3 && nowall) k_face<T, true, 3><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+    else if (ord == 3) k_face<T, false, 3><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+    else if (nowall) k_face<T, true><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+    else k_face<T><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
   }
   BCA_LAUNCHED();
   if (residu) {
@@ -483,11 +496,17 @@ int bca_volf(bca_ctx* c, const double* vol, double* volf) {
   return 0;
 }
 
-int bca_residual_dnc5(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+static int residual_impl(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p,
+                         bool nowall, int ord) {
   if (!c || !residu || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
-  int rc = ensure_work(c, c->w0, false);
+  int rc = ord == 5 ? need_gh3(c) : 0;
   if (rc) return rc;
-  return run_pipeline<double>(c, c->w0, w, residu, to_metric(g), to_phys(p));
+  if ((rc = ensure_work(c, c->w0, false))) return rc;
+  return run_pipeline<double>(c, c->w0, w, residu, to_metric(g), to_phys(p), nowall, ord);
+}
+
+int bca_residual_dnc5(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+  return residual_impl(c, residu, w, g, p, false, 5);
 }
 
 }  // extern "C"
@@ -538,6 +557,7 @@ int bca_residual_dnc5_noalloc(double* residu, const double* w, bca_metrics g, co
   const Mesh m = make_mesh(im, jm, gh);
   int rc = check_mesh(m);
   if (rc) return rc;
+  if (gh != 3) return fail(-1, "the 5th-order scheme needs gh = 3");
   const long need = bca_residual_noalloc_bytes(im, jm, gh);
   if (need > (long)BCA_NOALLOC_SCRATCH_BYTES)
     return fail(-4, "bca_residual_dnc5_noalloc: the mesh needs " + std::to_string(need) +
@@ -558,7 +578,7 @@ int bca_residual_dnc5_noalloc(double* residu, const double* w, bca_metrics g, co
   const dim3 b(64, 4);
   k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, mg, prim, grad);
   BCA_LAUNCHED();
-  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, grad);
+  k_grad_ghost<double><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, grad);
   BCA_LAUNCHED();
   const dim3 bf(64, 4);
   k_face<double><<<grid2(m.im + 1, m.jm + 1, bf), bf, 0, s>>>(m, ph, mg, w, prim, grad, hn);
@@ -568,20 +588,61 @@ int bca_residual_dnc5_noalloc(double* residu, const double* w, bca_metrics g, co
   return 0;
 }
 
-int bca_jvp_dnc5(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
-                 bca_metrics g, const bca_phys* p) {
+static int jvp_impl(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
+                    bca_metrics g, const bca_phys* p, bool nowall, int ord) {
   if (!c || !w || !wd || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
-  int rc = ensure_work(c, c->w1, true);
+  int rc = ord == 5 ? need_gh3(c) : 0;
   if (rc) return rc;
+  if ((rc = ensure_work(c, c->w1, true))) return rc;
   const Mesh& m = c->m;
   const long n = 5 * m.ncell;
   k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
   BCA_LAUNCHED();
-  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, c->w1.res, to_metric(g), to_phys(p)))) return rc;
+  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, c->w1.res, to_metric(g), to_phys(p), nowall, ord)))
+    return rc;
   const dim3 b(64, 4);
   k_unpack_interior<1><<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w1.res, residu, residud);
   BCA_LAUNCHED();
   return 0;
+}
+
+int bca_jvp_dnc5(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
+                 bca_metrics g, const bca_phys* p) {
+  return jvp_impl(c, residu, residud, w, wd, g, p, false, 5);
+}
+
+int bca_residual_dnc5_nowall(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+  return residual_impl(c, residu, w, g, p, true, 5);
+}
+
+// the reference's tangents of the variants leave residu as given
+int bca_jvp_dnc5_nowall(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
+                        bca_metrics g, const bca_phys* p) {
+  (void)residu;
+  if (!residud) return fail(-1, "null argument");
+  return jvp_impl(c, nullptr, residud, w, wd, g, p, true, 5);
+}
+
+int bca_residual_dnc3(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+  return residual_impl(c, residu, w, g, p, false, 3);
+}
+
+int bca_residual_dnc3_nowall(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p) {
+  return residual_impl(c, residu, w, g, p, true, 3);
+}
+
+int bca_jvp_dnc3(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd, bca_metrics g,
+                 const bca_phys* p) {
+  (void)residu;
+  if (!residud) return fail(-1, "null argument");
+  return jvp_impl(c, nullptr, residud, w, wd, g, p, false, 3);
+}
+
+int bca_jvp_dnc3_nowall(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
+                        bca_metrics g, const bca_phys* p) {
+  (void)residu;
+  if (!residud) return fail(-1, "null argument");
+  return jvp_impl(c, nullptr, residud, w, wd, g, p, true, 3);
 }
 
 static int bc_entry(bca_ctx* c, double* w, double* wd, const bca_bc_op& op, const double* nx,
@@ -835,7 +896,7 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BCA_LAUNCHED();
   k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, gm, c->w0.prim, c->w0.grad);
   BCA_LAUNCHED();
-  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
+  k_grad_ghost<double><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
   // phase 1: face-local blocks (independent of the zones).  The edge rows
   // (i-face rows 1-2, j-face rows 1-3) write face rows the interior launches
@@ -923,6 +984,7 @@ int bca_jacobian_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_p
                          const bca_bc_op* ops, int nops, const double* coefdiag, int withjn,
                          double* jac, int* ia, int* ja, long nbentry, int write_index) {
   if (!c) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   const Zones z{1, {1, 0}, {c->m.im, 0}, withjn != 0};
   return jacobian_impl(c, w, g, p, ops, nops, coefdiag, z, jac, ia, ja, nbentry, write_index);
 }
@@ -931,6 +993,7 @@ int bca_jacobian_colored_2zones(bca_ctx* c, const double* w, bca_metrics g, cons
                                 const bca_bc_op* ops, int nops, const double* coefdiag, int isplit,
                                 double* jac, int* ia, int* ja, long nbentry, int write_index) {
   if (!c) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   const int step = 2 * c->m.gh + 1;
   if (isplit < step || c->m.im - isplit < step) return fail(-1, "each zone must be at least 2gh+1 cells wide");
   const Zones z{2, {1, isplit + 1}, {isplit, c->m.im}, 1};
@@ -939,6 +1002,7 @@ int bca_jacobian_colored_2zones(bca_ctx* c, const double* w, bca_metrics g, cons
 
 int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_metrics g, const bca_phys* p) {
   if (!c || !residub || !wb || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   const Mesh& m = c->m;
   int rc = 0;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
@@ -954,7 +1018,7 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   BCA_LAUNCHED();
   k_grad<double><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, gm, c->w0.prim, c->w0.grad);
   BCA_LAUNCHED();
-  k_grad_ghost<double><<<grid1(m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
+  k_grad_ghost<double><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
   // edge rows on the side streams while the interior runs (disjoint face
   // rows of vjpi / vjpj; residub is only read until the join)
@@ -1089,6 +1153,7 @@ extern "C" {
 int bca_dz_5p(bca_ctx* c, double* dz, double* dz2, const double* w, const double* wd, bca_metrics g,
               const bca_phys* p) {
   if (!c || !w || !wd || !p || (!dz && !dz2)) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   if (dz && (!g.nx || !g.ny || !g.vol)) return fail(-1, "null metrics");
   const Mesh& m = c->m;
   const dim3 b(64, 4);
@@ -1112,6 +1177,7 @@ int bca_scatter_dz(bca_ctx* c, double* jac, int* ia, int* ja, const double* dz, 
 int bca_dz_colored(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
                    double* jdz, double* jdz2, int* ia, int* ja, long nbentry, int write_index) {
   if (!c || !w || !p || (!jdz && !jdz2) || (write_index && (!ia || !ja))) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   if (!g.nx || !g.ny || !g.vol) return fail(-1, "null metrics");
   if (nops < 0 || (nops > 0 && !ops)) return fail(-1, "bad boundary-condition list");
   const Mesh& m = c->m;
@@ -1399,6 +1465,7 @@ int jac_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const
             const double* coefdiag, int nzones, int zparam, double mini, int* indptr, int* indices, double* data,
             long cap, long* nnz, bool refresh, int* rebuilt) {
   if (!c || !indptr || !nnz || (cap > 0 && (!indices || !data))) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   const Mesh& m = c->m;
   const int step = 2 * m.gh + 1;
   Zones z;
@@ -1634,6 +1701,7 @@ extern "C" {
 int bca_autotune(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const bca_bc_op* ops, int nops,
                  int nzones, int zparam, double* jac, long nbentry, int reps, char* report, int report_len) {
   if (!c || !w || !p || !jac) return fail(-1, "null argument");
+  if (int rc_gh = need_gh3(c)) return rc_gh;
   if (reps < 1 || reps > 100) return fail(-1, "reps must be 1..100");
   const Mesh& m = c->m;
   const int step = 2 * m.gh + 1;

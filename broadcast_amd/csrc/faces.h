@@ -375,8 +375,37 @@ __device__ inline auto iface_viscous(const A& a) {
   }
 }
 
+// centred Euler flux and predictor of a face whose right cell is offset
+// (0, 0) and left cell (-DX, -DY): 6 points for ORD 5 (euler_o6_i.F,
+// predictor_7p_i.F), 4 points for ORD 3 (euler_o4_i.F, predictor_5p_i.F)
+template <int ORD, int DX, int DY, class A>
+__device__ inline auto euler_centred(const A& a, int e, double nxf, double nyf) {
+  if constexpr (ORD == 3)
+    return (Coef3::c1 * (efx<0, 0>(a, e) + efx<-DX, -DY>(a, e)) +
+            Coef3::c2 * (efx<DX, DY>(a, e) + efx<-2 * DX, -2 * DY>(a, e))) * nxf +
+           (Coef3::c1 * (efy<0, 0>(a, e) + efy<-DX, -DY>(a, e)) +
+            Coef3::c2 * (efy<DX, DY>(a, e) + efy<-2 * DX, -2 * DY>(a, e))) * nyf;
+  else
+    return (Coef::c1 * (efx<0, 0>(a, e) + efx<-DX, -DY>(a, e)) +
+            Coef::c2 * (efx<DX, DY>(a, e) + efx<-2 * DX, -2 * DY>(a, e)) +
+            Coef::c3 * (efx<2 * DX, 2 * DY>(a, e) + efx<-3 * DX, -3 * DY>(a, e))) * nxf +
+           (Coef::c1 * (efy<0, 0>(a, e) + efy<-DX, -DY>(a, e)) +
+            Coef::c2 * (efy<DX, DY>(a, e) + efy<-2 * DX, -2 * DY>(a, e)) +
+            Coef::c3 * (efy<2 * DX, 2 * DY>(a, e) + efy<-3 * DX, -3 * DY>(a, e))) * nyf;
+}
+template <int ORD, int DX, int DY, class A>
+__device__ inline auto predictor(const A& a, int e) {
+  if constexpr (ORD == 3)
+    return Coef3::d2 * a.template W<-2 * DX, -2 * DY>(e) - Coef3::d1 * a.template W<-DX, -DY>(e) +
+           Coef3::d1 * a.template W<0, 0>(e) - Coef3::d2 * a.template W<DX, DY>(e);
+  else
+    return -(Coef::d3 * a.template W<-3 * DX, -3 * DY>(e)) + Coef::d2 * a.template W<-2 * DX, -2 * DY>(e) -
+           Coef::d1 * a.template W<-DX, -DY>(e) + Coef::d1 * a.template W<0, 0>(e) -
+           Coef::d2 * a.template W<DX, DY>(e) + Coef::d3 * a.template W<2 * DX, 2 * DY>(e);
+}
+
 // hn = Euler - dissipation - viscous on the i-face between (i-1, j) and (i, j)
-template <bool O4, class A>
+template <bool O4, int ORD = 5, class A>
 __device__ inline void iface(const A& a, typename A::Out hn[5]) {
   using Out = typename A::Out;
   const double nxf = a.NX(0, 0, 0), nyf = a.NY(0, 0, 0);
@@ -387,13 +416,8 @@ __device__ inline void iface(const A& a, typename A::Out hn[5]) {
   const double nxl = nxf * inv, nyl = nyf * inv;
   sfor<0, 5>([&](auto E) {
     constexpr int e = decltype(E)::value;
-    auto fx = (Coef::c1 * (efx<0, 0>(a, e) + efx<-1, 0>(a, e)) + Coef::c2 * (efx<1, 0>(a, e) + efx<-2, 0>(a, e)) +
-               Coef::c3 * (efx<2, 0>(a, e) + efx<-3, 0>(a, e))) * nxf +
-              (Coef::c1 * (efy<0, 0>(a, e) + efy<-1, 0>(a, e)) + Coef::c2 * (efy<1, 0>(a, e) + efy<-2, 0>(a, e)) +
-               Coef::c3 * (efy<2, 0>(a, e) + efy<-3, 0>(a, e))) * nyf;
-    auto pred = -(Coef::d3 * a.template W<-3, 0>(e)) + Coef::d2 * a.template W<-2, 0>(e) -
-                Coef::d1 * a.template W<-1, 0>(e) + Coef::d1 * a.template W<0, 0>(e) -
-                Coef::d2 * a.template W<1, 0>(e) + Coef::d3 * a.template W<2, 0>(e);
+    auto fx = euler_centred<ORD, 1, 0>(a, e, nxf, nyf);
+    auto pred = predictor<ORD, 1, 0>(a, e);
     auto jump = 0.5 * (a.template W<0, 0>(e) - a.template W<-1, 0>(e));
     auto diss = ds.rspec * (ds.eps2 * jump + ds.eps4 * pred);
     if constexpr (e == 0) hn[0] = Out(fx - diss);
@@ -478,13 +502,16 @@ __device__ inline auto jface_viscous(const A& a) {
 // Euler j-flux of component e.  ROW: 0 = centred 6-point (rows j >= 4),
 // 2 = off-centred 3/2 face (row j = 2), 3 = off-centred 5/2 face (row j = 3);
 // the off-centred ones read cells of rows 1..5 (flux_num_dnc5.F90:164-196).
-template <int ROW, class A>
+// ORD 3 has one off-centred row, ROW 2, the mean of rows 1 and 2
+// (nearbndfluxes3demi_5p.F, flux_num_dnc3.F90:169-172).
+template <int ROW, int ORD = 5, class A>
 __device__ inline auto jeuler(const A& a, int e, double n2x, double n2y) {
   if constexpr (ROW == 0) {
-    return (Coef::c1 * (efx<0, 0>(a, e) + efx<0, -1>(a, e)) + Coef::c2 * (efx<0, 1>(a, e) + efx<0, -2>(a, e)) +
-            Coef::c3 * (efx<0, 2>(a, e) + efx<0, -3>(a, e))) * n2x +
-           (Coef::c1 * (efy<0, 0>(a, e) + efy<0, -1>(a, e)) + Coef::c2 * (efy<0, 1>(a, e) + efy<0, -2>(a, e)) +
-            Coef::c3 * (efy<0, 2>(a, e) + efy<0, -3>(a, e))) * n2y;
+    return euler_centred<ORD, 0, 1>(a, e, n2x, n2y);
+  } else if constexpr (ORD == 3) {
+    static_assert(ROW == 2, "the 3rd-order scheme has one off-centred row");
+    return (0.5 * efx<0, -1>(a, e) + 0.5 * efx<0, 0>(a, e)) * n2x +
+           (0.5 * efy<0, -1>(a, e) + 0.5 * efy<0, 0>(a, e)) * n2y;
   } else {
     constexpr int o = 1 - ROW;  // offset of row 1 relative to the face row
     constexpr double k0 = ROW == 2 ? 12.0 / 60.0 : -3.0 / 60.0;
@@ -500,7 +527,7 @@ __device__ inline auto jeuler(const A& a, int e, double n2x, double n2y) {
 }
 
 // j-face between (i, j-1) and (i, j) for rows j >= 2.
-template <int ROW, bool O4, class A>
+template <int ROW, bool O4, int ORD = 5, class A>
 __device__ inline void jface(const A& a, typename A::Out hn[5]) {
   using Out = typename A::Out;
   const double n2x = a.NX(0, 0, 1), n2y = a.NY(0, 0, 1);
@@ -511,10 +538,8 @@ __device__ inline void jface(const A& a, typename A::Out hn[5]) {
   const double nxl = n2x * inv, nyl = n2y * inv;
   sfor<0, 5>([&](auto E) {
     constexpr int e = decltype(E)::value;
-    auto fx = jeuler<ROW>(a, e, n2x, n2y);
-    auto pred = -(Coef::d3 * a.template W<0, -3>(e)) + Coef::d2 * a.template W<0, -2>(e) -
-                Coef::d1 * a.template W<0, -1>(e) + Coef::d1 * a.template W<0, 0>(e) -
-                Coef::d2 * a.template W<0, 1>(e) + Coef::d3 * a.template W<0, 2>(e);
+    auto fx = jeuler<ROW, ORD>(a, e, n2x, n2y);
+    auto pred = predictor<ORD, 0, 1>(a, e);
     auto jump = 0.5 * (a.template W<0, 0>(e) - a.template W<0, -1>(e));
     auto diss = ds.rspec * (ds.eps2 * jump + ds.eps4 * pred);
     if constexpr (e == 0) hn[0] = Out(fx - diss);
@@ -545,23 +570,37 @@ __device__ inline void jface_wall(const A& a, typename A::Out hn[5]) {
   hn[4] = Out(0.0);
 }
 
-// dispatch of the i-face / j-face variants on the row
-template <class A>
+// dispatch of the i-face / j-face variants on the row.  ORD 3 takes the
+// 2nd-order viscous flux everywhere and one off-centred Euler row
+// (flux_num_dnc3.F90:143-205).
+template <int ORD = 5, class A>
 __device__ inline void iface_any(const A& a, typename A::Out hn[5]) {
-  if (a.j >= 3) iface<true>(a, hn);
+  if constexpr (ORD == 3) iface<false, 3>(a, hn);
+  else if (a.j >= 3) iface<true>(a, hn);
   else iface<false>(a, hn);
 }
-template <class A>
+template <int ORD = 5, class A>
 __device__ inline void jface_any(const A& a, typename A::Out hn[5]) {
-  if (a.j >= 4) jface<0, true>(a, hn);
-  else if (a.j == 3) jface<3, true>(a, hn);
-  else if (a.j == 2) jface<2, false>(a, hn);
-  else jface_wall(a, hn);
+  if constexpr (ORD == 3) {
+    if (a.j >= 3) jface<0, false, 3>(a, hn);
+    else if (a.j == 2) jface<2, false, 3>(a, hn);
+    else jface_wall(a, hn);
+  } else {
+    if (a.j >= 4) jface<0, true>(a, hn);
+    else if (a.j == 3) jface<3, true>(a, hn);
+    else if (a.j == 2) jface<2, false>(a, hn);
+    else jface_wall(a, hn);
+  }
 }
 
 // One thread per (i, j) in [1, im+1] x [1, jm+1]: the i-face (i, j) when
 // j <= jm and the j-face (i, j) when i <= im (the faces the balance reads).
-template <class T>
+// NOWALL: the variants without wall rows (srcfv/rhs/flux_num_dnc5_nowall.F90:
+// 140-154, flux_num_dnc3_nowall.F90): every face takes the interior formulas,
+// the centred Euler flux and the viscous flux of the order (compact 4th-order
+// for ORD 5, 2nd-order for ORD 3), the ghost layers supplying the stencil at
+// the block edges.  ORD: 5 = flux_num_dnc5*, 3 = flux_num_dnc3*.
+template <class T, bool NOWALL = false, int ORD = 5>
 __global__ void __launch_bounds__(256) k_face(Mesh m, Phys ph, Metric g, const T* __restrict__ w,
                                               const T* __restrict__ prim, const T* __restrict__ grad,
                                               T* __restrict__ hn) {
@@ -574,12 +613,14 @@ __global__ void __launch_bounds__(256) k_face(Mesh m, Phys ph, Metric g, const T
   const long f = nidx(m, i, j);
   T h[5];
   if (j <= m.jm) {
-    iface_any(a, h);
+    if constexpr (NOWALL) iface<ORD == 5, ORD>(a, h);
+    else iface_any<ORD>(a, h);
 #pragma unroll
     for (int e = 0; e < 5; ++e) hn[f + (long)e * m.nnode] = h[e];
   }
   if (i <= m.im) {
-    jface_any(a, h);
+    if constexpr (NOWALL) jface<0, ORD == 5, ORD>(a, h);
+    else jface_any<ORD>(a, h);
 #pragma unroll
     for (int e = 0; e < 5; ++e) hn[f + (long)(5 + e) * m.nnode] = h[e];
   }

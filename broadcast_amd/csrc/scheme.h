@@ -49,6 +49,14 @@ enum { P_U = 0, P_V, P_W, P_T, P_P, P_MU, P_H, NPRIM };
 enum { G_UX = 0, G_UY, G_VX, G_VY, NGRAD };
 
 // scheme coefficients (srcfv/rhs/flux_num_dnc5.F90:65-118, coefnearbnd_7p.F)
+// of the 5th-order scheme; Coef3 holds those of the 3rd-order one
+// (srcfv/rhs/flux_num_dnc3.F90:103-119): 2nd-order gradient, 4-point centred
+// Euler flux and predictor.
+struct Coef3 {
+  static constexpr double b1 = 0.5;
+  static constexpr double c1 = 7.0 / 12.0, c2 = -1.0 / 12.0;
+  static constexpr double d1 = 3.0 / 12.0, d2 = 1.0 / 12.0;
+};
 struct Coef {
   static constexpr double b1 = 8.0 / 12.0, b2 = -1.0 / 12.0;            // 4th-order gradient
   static constexpr double c1 = 37.0 / 60.0, c2 = -8.0 / 60.0, c3 = 1.0 / 60.0;  // centred Euler
@@ -104,20 +112,24 @@ __global__ void k_prim(Mesh m, Phys ph, const T* __restrict__ w, T* __restrict__
 }
 
 // ---------------------------------------------------------------- gradients
-// gradop_5pi/j.F + gradient.F + geom/dxdy.F
-template <class T>
+// gradop_5pi/j.F (ORD 5) or gradop_3pi/j.F (ORD 3) + gradient.F + geom/dxdy.F
+template <int ORD = 5, class T>
 __device__ inline void gradient_at(const Mesh& m, const Metric& g, const T* prim, int i, int j,
                                    T out[4]) {
   const long n = m.ncell;
   const T* U = prim + P_U * n;
   const T* V = prim + P_V * n;
-  auto di = [&](const T* f) {
-    return Coef::b1 * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]) +
-           Coef::b2 * (f[cidx(m, i + 2, j)] - f[cidx(m, i - 2, j)]);
+  auto di = [&](const T* f) -> T {
+    if constexpr (ORD == 3) return Coef3::b1 * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]);
+    else
+      return Coef::b1 * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]) +
+             Coef::b2 * (f[cidx(m, i + 2, j)] - f[cidx(m, i - 2, j)]);
   };
-  auto dj = [&](const T* f) {
-    return Coef::b1 * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]) +
-           Coef::b2 * (f[cidx(m, i, j + 2)] - f[cidx(m, i, j - 2)]);
+  auto dj = [&](const T* f) -> T {
+    if constexpr (ORD == 3) return Coef3::b1 * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]);
+    else
+      return Coef::b1 * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]) +
+             Coef::b2 * (f[cidx(m, i, j + 2)] - f[cidx(m, i, j - 2)]);
   };
   T gui = di(U), gvi = di(V), guj = dj(U), gvj = dj(V);
   const long nn = m.nnode;
@@ -132,25 +144,35 @@ __device__ inline void gradient_at(const Mesh& m, const Metric& g, const T* prim
   out[G_VY] = dym1 * gvi + dym2 * gvj;
 }
 
-template <class T>
+template <class T, int ORD = 5>
 __global__ void k_grad(Mesh m, Metric g, const T* __restrict__ prim, T* __restrict__ grad) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > m.im || j > m.jm) return;
   T out[4];
-  gradient_at(m, g, prim, i, j, out);
+  gradient_at<ORD>(m, g, prim, i, j, out);
   const long c = cidx(m, i, j);
 #pragma unroll
   for (int k = 0; k < 4; ++k) grad[c + k * m.ncell] = out[k];
 }
 
 // gradveloingh.F: the faces that reach the residual read ghost gradients only
-// in columns 0 and im+1 (rows 1..jm) and in row jm+1 (columns 1..im); there
+// in columns 0 and im+1 (rows 1..jm), in row jm+1 and -- the variant without
+// wall rows, whose first j-faces are centred -- row 0 (columns 1..im); there
 // the reference's two-pass linear extrapolation reduces to 2 g(1) - g(2).
 template <class T>
 __global__ void k_grad_ghost(Mesh m, T* __restrict__ grad) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const long n = m.ncell;
+  if (t >= m.jm + m.im && t < m.jm + 2 * m.im) {
+    const int i = 1 + t - m.jm - m.im;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      T* G = grad + k * n;
+      G[cidx(m, i, 0)] = 2.0 * G[cidx(m, i, 1)] - G[cidx(m, i, 2)];
+    }
+    return;
+  }
   if (t < m.jm) {
     const int j = 1 + t;
 #pragma unroll

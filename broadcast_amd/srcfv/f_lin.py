@@ -7,16 +7,43 @@ from broadcast_amd import device as dev
 from broadcast_amd.srcfv import _bc
 
 
+def _tangent(entry, name, res, resd, w, wd, nx, ny, vol, volf, gh, phys, im, jm):
+    """resd = dR/dw . wd on the interior.  res is left as given: the generated
+    tangents assign only residud (their primal residual is never stored)."""
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(resd, w, wd, nx, ny, vol, volf, out=(0,))
+    g = _lib.Metrics(st[3].ptr, st[4].ptr, st[5].ptr, st[6].ptr)
+    ph = dev.phys(*phys)
+    _lib.check(getattr(_lib.lib(), entry)(ctx.h, None, st[0].ptr, st[1].ptr, st[2].ptr, g, ph), name)
+    dev.finish(st)
+
+
 def flux_num_dnc5_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,
                        rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
-    """srcfv/tangent/flux_num_dnc5_d.f90:15-3870: res and resd = dR/dw . wd."""
-    ctx = dev.context(im, jm, gh)
-    st = dev.stage(res, resd, w, wd, nx, ny, vol, volf, out=(0, 1))
-    g = _lib.Metrics(st[4].ptr, st[5].ptr, st[6].ptr, st[7].ptr)
-    ph = dev.phys(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4)
-    _lib.check(_lib.lib().bca_jvp_dnc5(ctx.h, st[0].ptr, st[1].ptr, st[2].ptr, st[3].ptr, g, ph),
-               "flux_num_dnc5_2d_d")
-    dev.finish(st)
+    """srcfv/tangent/flux_num_dnc5_d.f90:15-3870"""
+    _tangent("bca_jvp_dnc5", "flux_num_dnc5_2d_d", res, resd, w, wd, nx, ny, vol, volf, gh,
+             (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc5_nowall_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl,
+                              gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc5_nowall_d.f90:15"""
+    _tangent("bca_jvp_dnc5_nowall", "flux_num_dnc5_nowall_2d_d", res, resd, w, wd, nx, ny, vol, volf, gh,
+             (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc3_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,
+                       rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc3_d.f90:15, on arrays of ghost width gh = 2"""
+    _tangent("bca_jvp_dnc3", "flux_num_dnc3_2d_d", res, resd, w, wd, nx, ny, vol, volf, gh,
+             (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc3_nowall_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl,
+                              gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc3_nowall_d.f90:15, on arrays of ghost width gh = 2"""
+    _tangent("bca_jvp_dnc3_nowall", "flux_num_dnc3_nowall_2d_d", res, resd, w, wd, nx, ny, vol, volf, gh,
+             (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
 
 
 def bc_supandsubinlet_2d_d(w, wd0, loc, interf, field, nx, ny, gam, im, jm, lm=None, gh=None):

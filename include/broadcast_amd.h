@@ -111,10 +111,45 @@ int bca_residual_dnc5_noalloc(double *residu, const double *w, bca_metrics g, co
 
 /* ---- f_lin --------------------------------------------------------------- */
 /* flux_num_dnc5_2d_d(residu,residud,w,wd,...)
- * replaces srcfv/tangent/flux_num_dnc5_d.f90:15-3870: residu and
- * residud = dR/dw . wd on the interior (wd's ghosts as given). */
+ * replaces srcfv/tangent/flux_num_dnc5_d.f90:15-3870: residud = dR/dw . wd
+ * on the interior (wd's ghosts as given).  The reference's tangent leaves
+ * residu as given; here residu may be NULL (the drop-in passes NULL) or
+ * receives R(w) from the same pass. */
 int bca_jvp_dnc5(bca_ctx *ctx, double *residu, double *residud, const double *w,
                  const double *wd, bca_metrics g, const bca_phys *phys);
+
+/* flux_num_dnc5_nowall_2d(residu,w,...) replaces
+ * srcfv/rhs/flux_num_dnc5_nowall.F90:7-176 (the sensitivity card's scheme,
+ * card_bl2d_fv_npz_sens.py:87): the DNC5 residual with interior formulas on
+ * every face (no wall rows at j = 1), ghost layers closing the stencils. */
+int bca_residual_dnc5_nowall(bca_ctx *ctx, double *residu, const double *w, bca_metrics g,
+                             const bca_phys *phys);
+/* flux_num_dnc5_nowall_2d_d(residu,residud,w,wd,...) replaces
+ * srcfv/tangent/flux_num_dnc5_nowall_d.f90:15: residud = dR/dw . wd of the
+ * variant above on the interior.  As the reference's tangent does, it leaves
+ * residu as given (the argument may be NULL). */
+int bca_jvp_dnc5_nowall(bca_ctx *ctx, double *residu, double *residud, const double *w,
+                        const double *wd, bca_metrics g, const bca_phys *phys);
+
+/* The 3rd-order scheme, on a context of gh = 2 (the reference's
+ * gh = (order+1)/2, BROADCAST_npz.py:501-502): 2nd-order gradients, 4-point
+ * centred Euler flux and predictor, 2nd-order viscous flux, one off-centred
+ * Euler row next to the wall.
+ * flux_num_dnc3_2d(residu,w,...) replaces srcfv/rhs/flux_num_dnc3.F90:7-219;
+ * flux_num_dnc3_nowall_2d(residu,w,...) replaces
+ * srcfv/rhs/flux_num_dnc3_nowall.F90:7 (interior formulas on every face). */
+int bca_residual_dnc3(bca_ctx *ctx, double *residu, const double *w, bca_metrics g,
+                      const bca_phys *phys);
+int bca_residual_dnc3_nowall(bca_ctx *ctx, double *residu, const double *w, bca_metrics g,
+                             const bca_phys *phys);
+/* flux_num_dnc3_2d_d / flux_num_dnc3_nowall_2d_d(residu,residud,w,wd,...)
+ * replace srcfv/tangent/flux_num_dnc3_d.f90:15 and
+ * srcfv/tangent/flux_num_dnc3_nowall_d.f90:15: residud = dR/dw . wd on the
+ * interior; residu is left as given, as by the reference (may be NULL). */
+int bca_jvp_dnc3(bca_ctx *ctx, double *residu, double *residud, const double *w, const double *wd,
+                 bca_metrics g, const bca_phys *phys);
+int bca_jvp_dnc3_nowall(bca_ctx *ctx, double *residu, double *residud, const double *w,
+                        const double *wd, bca_metrics g, const bca_phys *phys);
 
 /* ---- pruned CSR of the colored COO ---------------------------------------- */
 /* remove_zero_jac (BROADCAST_npz.py:129-135) + scipy.sparse.csr_matrix of

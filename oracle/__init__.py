@@ -118,6 +118,30 @@ class Oracle:
                                        C.c_int(gh), _p(ph), C.c_int(im), C.c_int(jm))
 
     # f_bnd (primal) and f_lin (tangent, wd not None) --------------------------
+    def _scheme(self, name, arrays, gh, phys, im, jm):
+        args = [_p(a) for a in arrays] + [*self._i(gh), *self._r(*phys), *self._i(im, jm)]
+        _run_big_stack(lambda: getattr(self.lib, name + "_")(*args))
+
+    def flux_num_dnc3_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/rhs/flux_num_dnc3.F90:7"""
+        self._scheme("flux_num_dnc3_2d", (res, w, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_nowall_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/rhs/flux_num_dnc3_nowall.F90:7"""
+        self._scheme("flux_num_dnc3_nowall_2d", (res, w, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_2d_d(self, res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/tangent/flux_num_dnc3_d.f90"""
+        self._scheme("flux_num_dnc3_2d_d", (res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_nowall_2d_d(self, res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/tangent/flux_num_dnc3_nowall_d.f90"""
+        self._scheme("flux_num_dnc3_nowall_2d_d", (res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
     def bc_supandsubinlet_2d(self, w, loc, interf, field, nx, ny, gam, im, jm, gh=3, wd=None):
         lm = field.shape[0]
         self.lib.or_bc_inlet(_p(w), _p(wd), C.c_int(LOC[loc]), _p(_box(interf)), _p(field),
@@ -243,6 +267,46 @@ class Reference:
                 *self._r(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4),
                 *self._i(im, jm)]
         _run_big_stack(lambda: self.lib.flux_num_dnc5_2d_b_(*args))
+
+    def flux_num_dnc5_nowall_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv,
+                                prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+        args = [_p(res), _p(w), _p(x0), _p(y0), _p(nx), _p(ny), _p(xc), _p(yc), _p(vol),
+                _p(volf), *self._i(gh),
+                *self._r(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4),
+                *self._i(im, jm)]
+        _run_big_stack(lambda: self.lib.flux_num_dnc5_nowall_2d_(*args))
+
+    def flux_num_dnc5_nowall_2d_d(self, res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh,
+                                  cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+        args = [_p(res), _p(resd), _p(w), _p(wd), _p(x0), _p(y0), _p(nx), _p(ny), _p(xc),
+                _p(yc), _p(vol), _p(volf), *self._i(gh),
+                *self._r(cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4),
+                *self._i(im, jm)]
+        _run_big_stack(lambda: self.lib.flux_num_dnc5_nowall_2d_d_(*args))
+
+    def _scheme(self, name, arrays, gh, phys, im, jm):
+        args = [_p(a) for a in arrays] + [*self._i(gh), *self._r(*phys), *self._i(im, jm)]
+        _run_big_stack(lambda: getattr(self.lib, name + "_")(*args))
+
+    def flux_num_dnc3_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/rhs/flux_num_dnc3.F90:7"""
+        self._scheme("flux_num_dnc3_2d", (res, w, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_nowall_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/rhs/flux_num_dnc3_nowall.F90:7"""
+        self._scheme("flux_num_dnc3_nowall_2d", (res, w, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_2d_d(self, res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/tangent/flux_num_dnc3_d.f90"""
+        self._scheme("flux_num_dnc3_2d_d", (res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
+
+    def flux_num_dnc3_nowall_2d_d(self, res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
+        """srcfv/tangent/flux_num_dnc3_nowall_d.f90"""
+        self._scheme("flux_num_dnc3_nowall_2d_d", (res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf), gh,
+                     phys_im_jm[:11], *phys_im_jm[11:])
 
     def bc_supandsubinlet_2d(self, w, loc, interf, field, nx, ny, gam, im, jm, gh=3, wd=None):
         lm = field.shape[0]

@@ -10,8 +10,10 @@ repository root:
 
 `--dz-only` adds (or refreshes) the spanwise operators dz, dz2 of the case
 fixtures from their stored inputs without regenerating anything else;
-`--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b), and
-`--sbd-only` the outputs of set_bndbl_2d.
+`--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b),
+`--sbd-only` the outputs of set_bndbl_2d and `--nowall-only` the residual and
+tangent of the variant without wall rows (flux_num_dnc5_nowall_2d[_d]);
+`--dnc3-only` writes the 3rd-order scheme's fixtures (dnc3_*.npz, gh = 2).
 
 Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
 """
@@ -136,6 +138,68 @@ def sbd_vectors(R, fx):
     return dict(sbd_field=field, sbd_wbd=wbd)
 
 
+def nowall_vectors(R, fx):
+    """The variant without wall rows (srcfv/rhs/flux_num_dnc5_nowall.F90 and
+    its tangent srcfv/tangent/flux_num_dnc5_nowall_d.f90) on the stored state
+    after BCs and the first tangent direction after the linearised BCs:
+    res_nowall, resd_nowall."""
+    g = {k: np.asfortranarray(fx[f"geo_{k}"]) for k in ("x0", "y0", "nx", "ny", "xc", "yc", "vol", "volf")}
+    w = np.asfortranarray(fx["w_bc"])
+    wd = np.asfortranarray(fx["wd_bc"][0])
+    args = tuple(float(v) for v in fx["phys"])
+    gh, im, jm = int(fx["gh"]), int(fx["im"]), int(fx["jm"])
+    res = np.zeros_like(w, order="F")
+    R.flux_num_dnc5_nowall_2d(res, np.array(w, order="F", copy=True), g["x0"], g["y0"], g["nx"], g["ny"], g["xc"],
+                              g["yc"], g["vol"], g["volf"], gh, *args, im, jm)
+    r2 = np.zeros_like(w, order="F")
+    resd = np.zeros_like(w, order="F")
+    R.flux_num_dnc5_nowall_2d_d(r2, resd, np.array(w, order="F", copy=True), np.array(wd, order="F", copy=True),
+                                g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], gh,
+                                *args, im, jm)
+    # the generated tangent assigns only residud: residu is left as given
+    # (flux_num_dnc5_2d_d behaves the same way)
+    assert not r2.any()
+    return dict(res_nowall=res, resd_nowall=resd)
+
+
+def dnc3_fixture(R, case, seed):
+    """The 3rd-order scheme on a gh = 2 case (BROADCAST_npz.py:501-502):
+    flux_num_dnc3_2d and flux_num_dnc3_nowall_2d (srcfv/rhs) on the state after
+    the case's BCs, and their tangents (srcfv/tangent/flux_num_dnc3*_d.f90)
+    along one seeded direction after the linearised BCs."""
+    assert case.gh == 2
+    g = D.geometry(R, case)
+    w = np.array(case.w, order="F", copy=True)
+    D.apply_bcs(R, case, g, w)
+    rng = np.random.default_rng(seed)
+    gh, im, jm = case.gh, case.im, case.jm
+    d = np.zeros_like(w, order="F")
+    d[gh:-gh, gh:-gh, :] = rng.standard_normal((im, jm, 5))
+    wd_in = d.copy(order="F")
+    wcopy = np.array(w, order="F", copy=True)
+    D.apply_bcs(R, case, g, wcopy, wd=d)
+    args = case.phys.scheme_args()
+    out = {}
+    for tag, fn, fnd in (("", R.flux_num_dnc3_2d, R.flux_num_dnc3_2d_d),
+                         ("_nowall", R.flux_num_dnc3_nowall_2d, R.flux_num_dnc3_nowall_2d_d)):
+        res = np.zeros_like(w, order="F")
+        fn(res, np.array(w, order="F", copy=True), g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"],
+           g["vol"], g["volf"], gh, *args, im, jm)
+        r2 = np.zeros_like(w, order="F")
+        resd = np.zeros_like(w, order="F")
+        fnd(r2, resd, np.array(w, order="F", copy=True), np.array(d, order="F", copy=True), g["x0"], g["y0"],
+            g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], gh, *args, im, jm)
+        assert not r2.any()  # the generated tangent assigns residud only
+        out[f"res3{tag}"] = res
+        out[f"resd3{tag}"] = resd
+    return dict(kind=np.array(case.kind), im=im, jm=jm, gh=gh, phys=np.array(args),
+                x0_in=case.x0, y0_in=case.y0, w_in=case.w,
+                field=case.field if case.field is not None else np.zeros((0,)),
+                wbd=case.wbd, **{f"interf_{k}": v for k, v in case.interf.items()},
+                **{f"geo_{k}": v for k, v in g.items()},
+                w_bc=w, wd=wd_in, wd_bc=d, **out)
+
+
 def jac_fixture(R, case):
     g = D.geometry(R, case)
     w = np.array(case.w, order="F", copy=True)
@@ -155,10 +219,22 @@ def jac_fixture(R, case):
 CASES = ("bl24x16", "bl13x9", "cyl28x12")
 
 
+def DNC3_CASES(R):
+    return {"dnc3_bl20x12": dnc3_fixture(R, bl_case(20, 12, seed=6, gh=2), 16),
+            "dnc3_cyl21x10": dnc3_fixture(R, cylinder_case(21, 10, seed=7, gh=2), 17)}
+
+
 def main():
     R = oracle.Reference()
-    only = {"--dz-only": dz_vectors, "--bcb-only": bcb_vectors, "--sbd-only": sbd_vectors}
+    only = {"--dz-only": dz_vectors, "--bcb-only": bcb_vectors, "--sbd-only": sbd_vectors,
+            "--nowall-only": nowall_vectors}
     extra = [fn for flag, fn in only.items() if flag in sys.argv]
+    if "--dnc3-only" in sys.argv:
+        for name, fx in DNC3_CASES(R).items():
+            path = os.path.join(OUT, f"{name}.npz")
+            np.savez_compressed(path, **fx)
+            print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+        return
     if extra:
         for name in CASES:
             path = os.path.join(OUT, f"{name}.npz")
@@ -179,6 +255,8 @@ def main():
         fixtures[name].update(dz_vectors(R, fixtures[name]))
         fixtures[name].update(bcb_vectors(R, fixtures[name]))
         fixtures[name].update(sbd_vectors(R, fixtures[name]))
+        fixtures[name].update(nowall_vectors(R, fixtures[name]))
+    fixtures.update(DNC3_CASES(R))
     for name, fx in fixtures.items():
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **fx)

@@ -137,10 +137,134 @@ def test_dropin_modules_numpy_path(torch_mod, name):
         ok, err = close(wd, fx["wd_bc"][0], 1e-13)
         assert ok, err
         resd = np.zeros_like(w, order="F")
+        res_before = res.copy(order="F")
         f_lin.flux_num_dnc5_2d_d(res, resd, w, wd, fx["geo_x0"], fx["geo_y0"], geo["nx"], geo["ny"],
                                  geo["xc"], geo["yc"], geo["vol"], geo["volf"], gh, *args, im, jm)
         ok, err = close(resd[3:-3, 3:-3], fx["resd"][0][3:-3, 3:-3], 1e-12)
         assert ok, err
+        # the reference's tangent assigns residud only
+        assert np.array_equal(res, res_before)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_nowall_variant_matches_reference(torch_mod, name):
+    """flux_num_dnc5_nowall_2d and its tangent (srcfv/rhs/
+    flux_num_dnc5_nowall.F90:7-176, srcfv/tangent/flux_num_dnc5_nowall_d.f90:
+    15) through the drop-in f_sch / f_lin call forms, on the stored state after
+    BCs and the first linearised direction.  The tangent leaves residu as
+    given, as the reference's does."""
+    from broadcast_amd.srcfv import f_lin, f_sch
+
+    fx = load(name)
+    case = case_from(fx)
+    im, jm, gh = case.im, case.jm, case.gh
+    geo = {k: np.array(fx[f"geo_{k}"], order="F") for k in ("x0", "y0") + GEO}
+    args = case.phys.scheme_args()
+    w = np.array(fx["w_bc"], order="F")
+    res = np.zeros_like(w, order="F")
+    f_sch.flux_num_dnc5_nowall_2d(res, w, geo["x0"], geo["y0"], geo["nx"], geo["ny"], geo["xc"], geo["yc"],
+                                  geo["vol"], geo["volf"], gh, *args, im, jm)
+    ok, err = close(res[3:-3, 3:-3], fx["res_nowall"][3:-3, 3:-3], 1e-12)
+    assert ok, err
+    # the variant differs from the wall scheme on the rows next to j = 1
+    assert not np.allclose(res[3:-3, 3:5], fx["res"][3:-3, 3:5], rtol=1e-6, atol=0)
+    wd = np.array(fx["wd_bc"][0], order="F")
+    res2 = np.full_like(w, 7.0, order="F")
+    resd = np.zeros_like(w, order="F")
+    f_lin.flux_num_dnc5_nowall_2d_d(res2, resd, w, wd, geo["x0"], geo["y0"], geo["nx"], geo["ny"], geo["xc"],
+                                    geo["yc"], geo["vol"], geo["volf"], gh, *args, im, jm)
+    ok, err = close(resd[3:-3, 3:-3], fx["resd_nowall"][3:-3, 3:-3], 1e-12)
+    assert ok, err
+    assert (res2 == 7.0).all()
+
+
+@pytest.mark.parametrize("name", ["dnc3_bl20x12", "dnc3_cyl21x10"])
+def test_dnc3_schemes_match_reference(torch_mod, name):
+    """The 3rd-order scheme on gh = 2 arrays, through the drop-in call forms:
+    geometry, the case's BCs and their linearisation, flux_num_dnc3_2d and
+    flux_num_dnc3_nowall_2d (srcfv/rhs) and their tangents (srcfv/tangent/
+    flux_num_dnc3*_d.f90) against the reference's outputs."""
+    from broadcast_amd.srcfv import f_bnd, f_geom, f_lin, f_sch
+
+    fx = load(name)
+    case = case_from(fx)
+    im, jm, gh = case.im, case.jm, case.gh
+    assert gh == 2
+    g = {k: np.zeros_like(fx[f"geo_{k}"], order="F") for k in GEO}
+    x0, y0 = np.array(fx["x0_in"], order="F"), np.array(fx["y0_in"], order="F")
+    f_geom.computegeom_2d(x0, y0, g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], im, jm, gh)
+    if case.kind == "bl":
+        # the O-mesh's stored geometry takes periodic ghost metrics after
+        # computegeom (cylinder.py:499-527), which this call alone does not
+        for k in ("nx", "ny", "vol", "volf"):
+            ok, err = close(g[k], fx[f"geo_{k}"], 1e-14)
+            assert ok, (k, err)
+    geo = {k: np.array(fx[f"geo_{k}"], order="F") for k in ("x0", "y0") + GEO}
+    it = case.interf
+    gam = case.phys.gam
+    w = np.array(fx["w_in"], order="F")
+    wd = np.array(fx["wd"], order="F")
+    if case.kind == "bl":
+        f_bnd.bc_supandsubinlet_2d(w, "Ilo", it["Ilo"], case.field, geo["nx"], geo["ny"], gam, im, jm)
+        f_bnd.bc_no_reflexion_2d(w, case.wbd, "Jhi", it["Jhi"], geo["nx"], geo["ny"], gam, gh, im, jm)
+        f_bnd.bc_extrapolate_o2_2d(w, "Ihi", it["Ihi"], im, jm, gh)
+        f_bnd.bc_wall_viscous_adia_2d(w, "Jlo", gam, it["Jlo"], gh, im, jm)
+        w2 = np.array(w, order="F")
+        f_lin.bc_supandsubinlet_2d_d(w2, wd, "Ilo", it["Ilo"], case.field, geo["nx"], geo["ny"], gam, im, jm)
+        f_lin.bc_no_reflexion_2d_d(w2, wd, case.wbd, "Jhi", it["Jhi"], geo["nx"], geo["ny"], gam, gh, im, jm)
+        f_lin.bc_extrapolate_o2_2d_d(w2, wd, "Ihi", it["Ihi"], im, jm, gh)
+        f_lin.bc_wall_viscous_adia_2d_d(w2, wd, "Jlo", gam, it["Jlo"], gh, im, jm)
+        ok, err = close(wd, fx["wd_bc"], 1e-13)
+        assert ok, err
+    else:
+        from broadcast_amd.cases import join_boxes
+
+        f_bnd.bc_no_reflexion_2d(w, case.wbd, "Jhi", it["Jhi"], geo["nx"], geo["ny"], gam, gh, im, jm)
+        f_bnd.bc_wall_viscous_adia_2d(w, "Jlo", gam, it["Jlo"], gh, im, jm)
+        (prr1, prd2, tr), (prr2, prd1, _) = join_boxes(im, jm, gh)
+        for _ in range(2):
+            f_bnd.jn_match_2d(w, prr1, gh, gh, gh, gh, im, jm, w, prd2, gh, gh, gh, gh, im, jm, tr)
+            f_bnd.jn_match_2d(w, prr2, gh, gh, gh, gh, im, jm, w, prd1, gh, gh, gh, gh, im, jm, tr)
+        wd = np.array(fx["wd_bc"], order="F")
+    ok, err = close(w, fx["w_bc"], 1e-14)
+    assert ok, err
+    args = case.phys.scheme_args()
+    common = (geo["x0"], geo["y0"], geo["nx"], geo["ny"], geo["xc"], geo["yc"], geo["vol"], geo["volf"], gh,
+              *args, im, jm)
+    sl = (slice(gh, -gh), slice(gh, -gh))
+    for tag, fn, fnd in (("", f_sch.flux_num_dnc3_2d, f_lin.flux_num_dnc3_2d_d),
+                         ("_nowall", f_sch.flux_num_dnc3_nowall_2d, f_lin.flux_num_dnc3_nowall_2d_d)):
+        res = np.zeros_like(w, order="F")
+        fn(res, w, *common)
+        ok, err = close(res[sl], fx[f"res3{tag}"][sl], 1e-12)
+        assert ok, (tag, err)
+        res2 = np.full_like(w, 7.0, order="F")
+        resd = np.zeros_like(w, order="F")
+        fnd(res2, resd, w, np.array(fx["wd_bc"], order="F"), *common)
+        ok, err = close(resd[sl], fx[f"resd3{tag}"][sl], 1e-12)
+        assert ok, (tag, err)
+        assert (res2 == 7.0).all()
+
+
+def test_scheme_orders_refuse_wrong_ghost_width(torch_mod):
+    """The 5th-order entries and the Jacobian paths need gh = 3; on a gh = 2
+    context they fail with a message instead of reading past the arrays."""
+    import ctypes as C
+
+    from broadcast_amd import _lib
+    from broadcast_amd import device as dev
+
+    torch = torch_mod
+    ctx = dev.context(12, 10, 2)
+    L = _lib.lib()
+    n = (12 + 4) * (10 + 4) * 5
+    a = torch.zeros(n, dtype=torch.float64, device="cuda")
+    nn = (12 + 5) * (10 + 5) * 2
+    b = torch.ones(nn, dtype=torch.float64, device="cuda")
+    g = _lib.Metrics(b.data_ptr(), b.data_ptr(), a.data_ptr(), a.data_ptr())
+    ph = dev.phys(1004.5, 717.5, 0.72, 1.4, 287.0, 110.4, 1.7e-5, 273.0, 110.4, 0.0, 1.0)
+    rc = L.bca_residual_dnc5(ctx.h, a.data_ptr(), a.data_ptr(), g, ph)
+    assert rc != 0 and b"gh = 3" in L.bca_last_error()
 
 
 @pytest.mark.parametrize("name", ["jac_bl14x10", "jac_cyl14x8"])
