@@ -97,6 +97,7 @@ _SIGS = {
     "bca_gemm_bf16": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, C.c_long, _dp, C.c_long, C.c_long, _dp,
                       C.c_long, C.c_long, C.c_int, C.c_int],
     "bca_lu_np": [_dp, _dp, C.c_int, C.c_long, C.c_double, _dp, _dp],
+    "bca_inv_np": [_dp, _dp, C.c_int, C.c_long, _dp, C.c_double, _dp],
     "bca_gemm_f64_sub": [_dp, C.c_int, C.c_int, C.c_int, _dp, C.c_long, _dp, C.c_long, _dp, C.c_long],
     "bca_csr_spmv": [_dp, C.c_long, _dp, _dp, _dp, _dp, _dp],
 }

@@ -162,6 +162,160 @@ __global__ void __launch_bounds__(256) k_gemm_f64_sub(int M, int N, int K, const
       }
 }
 
+// ------------------------------------------------------------------ inverse
+// Inverse without pivoting of a dense n x n block by Gauss-Jordan steps on
+// GJ_NB-column panels, each step one launch from Ain to Aout (every entry
+// written, so the caller alternates two buffers).  For the panel kk =
+// k0..k0+nb-1 with pivot tile P = A(kk, kk):
+//
+//   Aout(kk, kk) = P^-1                      Aout(kk, j) = P^-1 A(kk, j)
+//   Aout(i, kk)  = -A(i, kk) P^-1            Aout(i, j)  = A(i, j) - A(i, kk) P^-1 A(kk, j)
+//
+// for i, j outside kk.  Every entry is one product out = A' - T U of a
+// GJ_T x GJ_T output tile, with T = A(i, kk) P^-1 (rows of kk: T = -P^-1)
+// and U = A(kk, j) (columns of kk: U = the identity), A' = A off the panel
+// rows and columns and 0 on them.  A workgroup (4 waves, one 64 x 64
+// quadrant each, v_mfma_f64_16x16x4f64) first issues the loads of its A'
+// tile into registers and of A(i, kk), A(kk, j) into LDS; meanwhile wave 0
+// inverts P in registers (lane r + 32 h holds row r, columns 16 h .. 16 h +
+// 15; each elimination step broadcasts the pivot row by lane shuffles).
+// Workgroup (0, 0) reports a pivot below tiny times the tile's largest
+// |entry| in *bad.  After n / GJ_NB steps Aout holds A^-1 (2 n^3 flops).
+// The Newton driver multiplies the off-diagonal blocks of its block row by
+// this inverse (one GEMM instead of two triangular solves) and applies it to
+// the right-hand sides of its solves.
+constexpr int GJ_NB = 32;
+constexpr int GJ_T = 128;
+__global__ void __launch_bounds__(256) k_gj_step(const double* __restrict__ Ain, long lda, double* __restrict__ Aout,
+                                                 long ldo, int n, int k0, int nb, double tiny, int* __restrict__ bad) {
+  __shared__ double P[GJ_NB][GJ_NB + 1];
+  // A(i, kk) of the tile's rows, overwritten by T (each wave reads and then
+  // writes only its own 32 rows)
+  __shared__ double AT[GJ_T][GJ_NB + 1];
+  __shared__ double U[GJ_NB][GJ_T + 1];
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int lr = lane % 16, lk = lane / 16;
+  const int i0 = blockIdx.y * GJ_T, j0 = blockIdx.x * GJ_T;
+  const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+  auto in_panel = [&](int g) { return g >= k0 && g < k0 + nb; };
+  // A' of this thread's output entries: D(lk + 4 t, lr) of MFMA tile (a, b)
+  double a0[4][4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int gi = i0 + wm + 16 * a + lk + 4 * t, gj = j0 + wn + 16 * b + lr;
+        a0[a][b][t] = (gi < n && gj < n && !in_panel(gi) && !in_panel(gj)) ? Ain[(long)gi * lda + gj] : 0.0;
+      }
+  for (int x = tid; x < GJ_T * GJ_NB; x += 256) {
+    const int r = x / GJ_NB, c = x % GJ_NB;
+    const int gi = i0 + r;
+    AT[r][c] = (gi < n && c < nb) ? Ain[(long)gi * lda + k0 + c] : 0.0;
+    const int q = x / GJ_T, cc = x % GJ_T;
+    const int gj = j0 + cc;
+    double u = 0.0;
+    if (in_panel(gj)) u = (q == gj - k0) ? 1.0 : 0.0;
+    else if (gj < n && q < nb) u = Ain[(long)(k0 + q) * lda + gj];
+    U[q][cc] = u;
+  }
+  if (wave == 0) {
+    const int r = lane % 32, h = lane / 32;
+    double p[16];
+    double amax = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = 16 * h + q;
+      const bool in = r < nb && c < nb;
+      p[q] = in ? Ain[(long)(k0 + r) * lda + k0 + c] : (r == c ? 1.0 : 0.0);
+      amax = fmax(amax, in ? fabs(p[q]) : 0.0);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off));
+    const double thr = tiny * amax;
+#pragma unroll
+    for (int k = 0; k < GJ_NB; ++k) {
+      const int kh = k / 16, kq = k % 16;
+      double rowk[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) rowk[q] = __shfl(p[q], k + 32 * h);
+      const double piv = __shfl(p[kq], k + 32 * kh);
+      const double f = __shfl(p[kq], r + 32 * kh);
+      if (k < nb && blockIdx.x == 0 && blockIdx.y == 0 && lane == 0 && !(fabs(piv) > thr)) *bad = 1;
+      const double pinv = 1.0 / piv;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = 16 * h + q;
+        const double nk = rowk[q] * pinv;
+        if (r == k) p[q] = (c == k) ? pinv : nk;
+        else p[q] = (c == k) ? -f * pinv : p[q] - f * nk;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) P[r][16 * h + q] = p[q];
+  }
+  __syncthreads();
+  // T = A(i, kk) P^-1: wave w rows 32 w .. 32 w + 31
+  {
+    f64x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < GJ_NB; k += 4) {
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = AT[32 * wave + 16 * a + lr][k + lk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = P[k + lk][16 * b + lr];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int r = 32 * wave + 16 * a + lk + 4 * t, c = 16 * b + lr;
+          const int gi = i0 + r;
+          AT[r][c] = in_panel(gi) ? -P[gi - k0][c] : acc[a][b][t];
+        }
+  }
+  __syncthreads();
+  // out = A' - T U over the wave's 64 x 64 quadrant, 4 x 4 MFMA tiles
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < GJ_NB; k += 4) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) av[a] = AT[wm + 16 * a + lr][k + lk];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) bv[b] = U[k + lk][wn + 16 * b + lr];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int gi = i0 + wm + 16 * a + lk + 4 * t, gj = j0 + wn + 16 * b + lr;
+        if (gi < n && gj < n) Aout[(long)gi * ldo + gj] = a0[a][b][t] - acc[a][b][t];
+      }
+}
+
 // y = A x for a CSR matrix (one row per thread).
 __global__ void k_csr_spmv(long n, const int* __restrict__ indptr, const int* __restrict__ indices,
                            const double* __restrict__ data, const double* __restrict__ x, double* __restrict__ y) {

@@ -1621,6 +1621,29 @@ int bca_lu_np(void* stream, double* A, int n, long ld, double tiny, int* bad, do
   return 0;
 }
 
+int bca_inv_np(void* stream, double* A, int n, long ld, double* work, double tiny, int* bad) {
+  if (!A || !bad || !work) return fail(-1, "null argument");
+  if (n < 1 || ld < n) return fail(-1, "bca_inv_np: need n >= 1 and ld >= n");
+  const hipStream_t s = (hipStream_t)stream;
+  const dim3 g((n + GJ_T - 1) / GJ_T, (n + GJ_T - 1) / GJ_T);
+  double* buf[2] = {A, work};
+  const long ldb[2] = {ld, (long)n};
+  int cur = 0;
+  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
+    const int nb = std::min(GJ_NB, n - k0);
+    k_gj_step<<<g, 256, 0, s>>>(buf[cur], ldb[cur], buf[1 - cur], ldb[1 - cur], n, k0, nb, tiny, bad);
+    BCA_LAUNCHED();
+    cur = 1 - cur;
+  }
+  if (cur == 1) {
+    // an odd number of steps leaves the inverse in work
+    if (hipMemcpy2DAsync(A, ld * sizeof(double), work, (size_t)n * sizeof(double), (size_t)n * sizeof(double), n,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(-3, "bca_inv_np: copy back failed");
+  }
+  return 0;
+}
+
 int bca_gemm_f64_sub(void* stream, int M, int N, int K, const double* A, long lda, const double* B, long ldb,
                      double* C, long ldc) {
   if (!A || !B || !C) return fail(-1, "null argument");

@@ -58,12 +58,18 @@ class BlockBandLU:
     block first and factors it on a side stream while the remaining Schur
     products of step I run on the current stream."""
 
-    def __init__(self, im, jm, outer="i", kb=None, group=1, lookahead=True, pivot=None):
+    def __init__(self, im, jm, outer="i", kb=None, group=1, lookahead=True, pivot=None, diag="inv"):
         self.t = dev.torch()
-        # pivot=None: the project's LU without pivoting on the device (bca_lu_np,
-        # FP64 matrix cores) for the diagonal blocks; torch's pivoted LU on the
-        # CPU or when asked (pivot=True) or when a pivot comes out too small
+        # pivot=None: the project's elimination without pivoting on the device
+        # for the diagonal blocks -- diag="inv": their inverses by Gauss-Jordan
+        # (bca_inv_np), the block row and the solves multiplied by them;
+        # diag="lu": their LU factors (bca_lu_np) and triangular solves.
+        # torch's pivoted LU on the CPU or when asked (pivot=True) or when a
+        # pivot comes out too small.
         self.pivot = pivot
+        if diag not in ("inv", "lu"):
+            raise ValueError("diag must be 'inv' or 'lu'")
+        self.diag = diag
         self.im, self.jm = im, jm
         self.outer = outer
         self.n = 5 * im * jm
@@ -183,10 +189,14 @@ class BlockBandLU:
         return self
 
     def np_factor(self, R, F):
-        """Block elimination with the project's unpivoted LU of each diagonal
-        block (bca_lu_np) and triangular solves / Schur products by BLAS;
-        the next diagonal block is factored on a side stream while the rest
-        of the step's Schur products run.  False if a pivot was too small."""
+        """Block elimination with the project's unpivoted kernels for the
+        diagonal blocks: with diag="inv" each diagonal block is replaced by its
+        inverse (bca_inv_np) and the block row X = D^-1 A(I, J) is one GEMM;
+        with diag="lu" by its LU factors (bca_lu_np) and X comes from two
+        triangular solves.  The first block of X and the next diagonal
+        block's update go first; that block is then inverted/factored on a
+        side stream while the rest of the step's products run.  False if a
+        pivot was too small."""
         import ctypes as C
         from . import _lib
 
@@ -197,16 +207,32 @@ class BlockBandLU:
         cur = t.cuda.current_stream()
         side = t.cuda.Stream() if self.lookahead else cur
         self._bad.zero_()
-        # one 32 x 32 scratch tile per stream the factorizations run on
-        work = [t.empty(32 * 32, dtype=t.float64, device=R.device) for _ in range(2)]
+        inv = self.diag == "inv"
+        # scratch of each stream the diagonal blocks are processed on
+        wsize = B * B if inv else 32 * 32
+        work = [t.empty(wsize, dtype=t.float64, device=R.device) for _ in range(2 if self.lookahead else 1)]
 
-        def lu_np(I, stream):
+        def diag_np(I, stream):
             D = R[I, :, kb * B:(kb + 1) * B]
-            _lib.check(L_.bca_lu_np(C.c_void_p(stream.cuda_stream), C.c_void_p(D.data_ptr()), B, W, 1e-14,
-                                    C.c_void_p(self._bad.data_ptr()), C.c_void_p(work[stream is side].data_ptr())),
-                       "bca_lu_np")
+            wk = work[1 if (self.lookahead and stream is side) else 0]
+            sp = C.c_void_p(stream.cuda_stream)
+            if inv:
+                rc = L_.bca_inv_np(sp, C.c_void_p(D.data_ptr()), B, W, C.c_void_p(wk.data_ptr()), 1e-14,
+                                   C.c_void_p(self._bad.data_ptr()))
+            else:
+                rc = L_.bca_lu_np(sp, C.c_void_p(D.data_ptr()), B, W, 1e-14, C.c_void_p(self._bad.data_ptr()),
+                                  C.c_void_p(wk.data_ptr()))
+            _lib.check(rc, "bca_inv_np" if inv else "bca_lu_np")
 
-        lu_np(0, cur)
+        def apply_inv(D, X):
+            """X <- D^-1 X in place."""
+            if inv:
+                X.copy_(D @ X)
+            else:
+                X.copy_(t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, X, upper=False,
+                                                                               unitriangular=True), upper=True))
+
+        diag_np(0, cur)
         ready = None
         for I in range(self.nb):
             if ready is not None:
@@ -217,21 +243,21 @@ class BlockBandLU:
                 continue
             D = R[I, :, kb * B:(kb + 1) * B]
             Xr = R[I, :, (kb + 1) * B:(kb + 1 + hi - I) * B]
-            Xr.copy_(t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, Xr, upper=False, unitriangular=True),
-                                               upper=True))
-            if I == 0 and self.nfar:
-                F.copy_(t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, F, upper=False, unitriangular=True),
-                                                  upper=True))
+            apply_inv(D, Xr[:, :B])
             R[I + 1, :, kb * B:(kb + 1) * B].sub_(R[I + 1, :, (kb - 1) * B:kb * B] @ Xr[:, :B])
             if self.lookahead:
                 ev = t.cuda.Event()
                 ev.record(cur)
                 side.wait_event(ev)
-                lu_np(I + 1, side)
+                diag_np(I + 1, side)
                 ready = t.cuda.Event()
                 ready.record(side)
             else:
-                lu_np(I + 1, cur)
+                diag_np(I + 1, cur)
+            if hi - I > 1:
+                apply_inv(D, Xr[:, B:])
+            if I == 0 and self.nfar:
+                apply_inv(D, F)
             for dd in range(1, hi - I + 1):
                 K2 = I + dd
                 L = R[K2, :, (kb - dd) * B:(kb - dd + 1) * B]
@@ -256,6 +282,8 @@ class BlockBandLU:
             return t.linalg.lu_solve(self.lu[I], self.piv[I], v.unsqueeze(1) if v.dim() == 1 else v).reshape(v.shape)
         kb, B = self.kb, self.B
         D = self.R[I, :, kb * B:(kb + 1) * B]
+        if self.diag == "inv":
+            return D @ v
         m = v.unsqueeze(1) if v.dim() == 1 else v
         y = t.linalg.solve_triangular(D, t.linalg.solve_triangular(D, m, upper=False, unitriangular=True), upper=True)
         return y.reshape(v.shape)

@@ -357,9 +357,15 @@ int bca_jacobian_colored_csr_refresh(bca_ctx *ctx, const double *w, bca_metrics 
  * trailing updates on the FP64 matrix cores.  *bad (device int, set by the
  * caller to 0) becomes 1 if a pivot falls below tiny times its tile's
  * largest entry; work: device scratch of 32 x 32 doubles.
+ * bca_inv_np: in-place inverse without pivoting of the n x n row-major block
+ * at A (ld) by Gauss-Jordan steps on 32-column panels (each step a product
+ * on the FP64 matrix cores over the whole block); the block-banded driver
+ * multiplies its off-diagonal blocks and right-hand sides by it.  work:
+ * device scratch of n x n doubles; *bad as for bca_lu_np.
  * bca_gemm_f64_sub: C -= A B, row-major, FP64 matrix cores.
  * bca_csr_spmv: y = A x for a CSR matrix (the residual of a refined solve). */
 int bca_lu_np(void *hip_stream, double *A, int n, long ld, double tiny, int *bad, double *work);
+int bca_inv_np(void *hip_stream, double *A, int n, long ld, double *work, double tiny, int *bad);
 int bca_gemm_f64_sub(void *hip_stream, int M, int N, int K, const double *A, long lda, const double *B,
                      long ldb, double *C, long ldc);
 int bca_csr_spmv(void *hip_stream, long n, const int *indptr, const int *indices, const double *data,

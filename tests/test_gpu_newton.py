@@ -202,13 +202,50 @@ def test_lu_np_reproduces_the_block(torch_mod, n):
     assert int(bad.item()) == 1
 
 
+@pytest.mark.parametrize("n", [37, 200, 1001])
+def test_inv_np_inverts_the_block(torch_mod, n):
+    """bca_inv_np (Gauss-Jordan without pivoting, ping-pong through work) on a
+    block of the Newton matrices' kind inside a wider array: A^-1 A = I to
+    1e-13 normwise, torch's pivoted inverse agrees, the columns outside the
+    block stay untouched; a zero leading pivot is reported.  n = 37 takes two
+    panel steps, 200 seven (the result comes back from work), 1001 32."""
+    import ctypes as C
+    from broadcast_amd import _lib
+
+    t = torch_mod
+    g = t.Generator(device="cuda").manual_seed(n + 1)
+    A0 = t.randn(n, n, dtype=t.float64, device="cuda", generator=g) + 4 * n ** 0.5 * t.eye(n, dtype=t.float64,
+                                                                                     device="cuda")
+    big = t.zeros(n, n + 17, dtype=t.float64, device="cuda")
+    big[:, 7:7 + n] = A0
+    bad = t.zeros(1, dtype=t.int32, device="cuda")
+    work = t.empty(n * n, dtype=t.float64, device="cuda")
+    s = t.cuda.current_stream().cuda_stream
+    Av = big[:, 7:7 + n]
+    _lib.check(_lib.lib().bca_inv_np(C.c_void_p(s), C.c_void_p(Av.data_ptr()), n, big.stride(0),
+                                     C.c_void_p(work.data_ptr()), 1e-14, C.c_void_p(bad.data_ptr())), "bca_inv_np")
+    eye = t.eye(n, dtype=t.float64, device="cuda")
+    assert int(bad.item()) == 0
+    assert float((Av @ A0 - eye).norm() / n ** 0.5) < 1e-13
+    ref = t.linalg.inv(A0)
+    assert float((Av - ref).norm() / ref.norm()) < 1e-13
+    assert t.equal(big[:, :7], t.zeros_like(big[:, :7])) and t.equal(big[:, 7 + n:], t.zeros_like(big[:, 7 + n:]))
+    Z = A0.clone()
+    Z[0, 0] = 0.0
+    _lib.check(_lib.lib().bca_inv_np(C.c_void_p(s), C.c_void_p(Z.data_ptr()), n, n, C.c_void_p(work.data_ptr()),
+                                     1e-14, C.c_void_p(bad.data_ptr())), "bca_inv_np")
+    assert int(bad.item()) == 1
+
+
+@pytest.mark.parametrize("diag", ["inv", "lu"])
 @pytest.mark.parametrize("mk,outer", [(lambda: bl_case(40, 24, seed=71), "i"),
                                       (lambda: cylinder_case(48, 32, seed=72), "j")])
-def test_block_band_lu_on_device_matches_dense_solve(torch_mod, mk, outer):
-    """The Newton driver's solve on the device (unpivoted project LU of the
-    diagonal blocks, far blocks outside the band, look-ahead) on the colored
-    Jacobian from the GPU path, refined with the CSR residual: the same
-    solution as torch's pivoted dense solve to 1e-10, residual <= 1e-13."""
+def test_block_band_lu_on_device_matches_dense_solve(torch_mod, mk, outer, diag):
+    """The Newton driver's solve on the device (diagonal blocks inverted by
+    Gauss-Jordan or factored by LU, both unpivoted project kernels; far
+    blocks outside the band; look-ahead) on the colored Jacobian from the
+    GPU path, refined with the CSR residual: the same solution as torch's
+    pivoted dense solve to 1e-10, residual <= 1e-13."""
     from broadcast_amd import newton as N
     from broadcast_amd.flow import Flow, to_device
 
@@ -223,7 +260,7 @@ def test_block_band_lu_on_device_matches_dense_solve(torch_mod, mk, outer):
     A = t.sparse_csr_tensor(indptr.long(), indices.long(), data, size=(n, n)).to_dense()
     b = t.randn(n, dtype=t.float64, device="cuda", generator=t.Generator(device="cuda").manual_seed(3))
     ref = t.linalg.solve(A, b)
-    lu = N.BlockBandLU(case.im, case.jm, outer=outer).factor(indptr, indices, data)
+    lu = N.BlockBandLU(case.im, case.jm, outer=outer, diag=diag).factor(indptr, indices, data)
     assert lu.np_lu and lu.kb == 3
     x = lu.solve(b)
     k, rel = N.refine(lu, (indptr, indices, data), b, x)
