@@ -255,23 +255,57 @@ def freestream(flow):
     return to_device(flow.case.w)
 
 
-def baseflows(im, jm, params_phys, tol=1e-10, log=None):
-    """Newton baseflows from the free stream at each (Re, M): (departures
-    from the free stream, N x n FP64 device tensor, list of Solve, the (Re, M)
-    pairs that converged -- a pair that does not is left out and logged)."""
+def continuation_solve(im, jm, re, mach, tol=1e-10, ite=12, re_start=RE_RANGE[0], log=None):
+    """Continuation Newton in Re (broadcast_amd.newton.continuation) from the
+    free stream at re_start to re at fixed M: the fallback when Newton from
+    a start does not converge.  Returns (Solve, flow, w)."""
+    from .newton import continuation
+
+    t = dev.torch()
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, w, hist = continuation(lambda r: make_flow(im, jm, r, mach), float(re), float(re_start), steps=4, ite=ite,
+                              cfl=1e21, tol=tol)
+    t.cuda.synchronize()
+    its = sum(len(lg.norms) - 1 for _, lg in hist)
+    if log is not None:
+        log(f"continuation Re {re_start:.0f} -> {re:.1f}: {len(hist)} solves, {its} Newton steps")
+    return Solve(time.perf_counter() - t0, its, True, "continuation", hist[-1][1].norms), f, w
+
+
+def baseflows(im, jm, params_phys, tol=1e-10, log=None, warm=True, ite=12):
+    """Newton baseflows at each (Re, M): (departures from the free stream,
+    N x n FP64 device tensor, list of Solve, the (Re, M) pairs).  With warm,
+    the points are taken in order of Re and each starts from the converged
+    baseflow nearest in scaled (Re, M) (the free stream for the first); a
+    start that does not converge in `ite` steps falls back to continuation
+    in Re.  A pair that converges in neither way is left out and logged."""
     t = dev.torch()
     out, stats, used = [], [], []
-    for k, (re, ma) in enumerate(params_phys):
+    order = np.argsort(np.asarray(params_phys)[:, 0]) if warm else np.arange(len(params_phys))
+    for k in order:
+        re, ma = params_phys[k]
         f = make_flow(im, jm, re, ma)
         w = freestream(f)
         free = interior_field(f, w)
-        s = solve(f, w, tol=tol)
-        s.start = "free stream"
-        if log is not None:
-            log(f"baseflow {k}: Re {re:.1f} M {ma:.3f} {s.iterations} Newton steps {s.seconds:.2f} s"
-                f"{'' if s.converged else ' NOT CONVERGED, left out'}")
+        start = "free stream"
+        if warm and used:
+            d = np.linalg.norm(scale_params(used) - scale_params([[re, ma]]), axis=1)
+            near = int(np.argmin(d))
+            set_interior(f, w, free + out[near])
+            start = f"baseflow at Re {used[near][0]:.1f} M {used[near][1]:.3f}"
+        s = solve(f, w, tol=tol, ite=ite)
+        s.start = start
         if not s.converged:
-            continue
+            try:
+                s, f, w = continuation_solve(im, jm, re, ma, tol=tol, ite=ite, log=log)
+            except RuntimeError as err:
+                if log is not None:
+                    log(f"baseflow Re {re:.1f} M {ma:.3f}: {err}, left out")
+                continue
+        if log is not None:
+            log(f"baseflow {len(used)}: Re {re:.1f} M {ma:.3f} from {s.start}: {s.iterations} Newton steps "
+                f"{s.seconds:.2f} s")
         out.append(interior_field(f, w) - free)
         stats.append(s)
         used.append((re, ma))
@@ -285,10 +319,10 @@ def guess_field(flow, net, re, mach):
     return interior_field(flow, freestream(flow)) + net.predict([[re, mach]])[0]
 
 
-def solve_from_guess(flow, guess, tol=1e-10, ite=40):
-    """Newton from the learned guess (interior set, ghosts by the BCs); the
-    free-stream start is the fallback when that does not converge.  The time
-    includes the fallback's."""
+def solve_from_guess(flow, guess, tol=1e-10, ite=12):
+    """Newton from the learned guess (interior set, ghosts by the BCs);
+    continuation Newton in Re is the fallback when that does not converge.
+    The time includes the fallback's."""
     t = dev.torch()
     t.cuda.synchronize()
     t0 = time.perf_counter()
@@ -297,8 +331,8 @@ def solve_from_guess(flow, guess, tol=1e-10, ite=40):
     s = solve(flow, w, tol=tol, ite=ite)
     s.start = "learned"
     if not s.converged or not all(np.isfinite(s.norms)):
-        w = freestream(flow)
-        s2 = solve(flow, w, tol=tol, ite=ite)
+        ph = flow.case.phys
+        s2, _, w = continuation_solve(flow.im, flow.jm, ph.reynolds, ph.mach, tol=tol, ite=ite)
         s = Solve(time.perf_counter() - t0, s.iterations + s2.iterations, s2.converged, "learned, fell back",
                   s.norms + s2.norms)
     else:
@@ -306,11 +340,31 @@ def solve_from_guess(flow, guess, tol=1e-10, ite=40):
     return s, w
 
 
+def solve_plain(flow, tol=1e-10, ite=12):
+    """The plain method: Newton from the free stream, continuation Newton in
+    Re when that does not converge in `ite` steps (time included)."""
+    t = dev.torch()
+    t.cuda.synchronize()
+    t0 = time.perf_counter()
+    w = freestream(flow)
+    s = solve(flow, w, tol=tol, ite=ite)
+    s.start = "free stream"
+    if not s.converged or not all(np.isfinite(s.norms)):
+        ph = flow.case.phys
+        s2, _, w = continuation_solve(flow.im, flow.jm, ph.reynolds, ph.mach, tol=tol, ite=ite)
+        s = Solve(time.perf_counter() - t0, s.iterations + s2.iterations, s2.converged, "free stream, fell back",
+                  s.norms + s2.norms)
+    else:
+        s.seconds = time.perf_counter() - t0
+    return s, w
+
+
 # ------------------------------------------------------------------ experiment
-def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, seed=4, log=print):
+def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, seed=4, log=print, progress=None):
     """Baseflows at n_train Sobol points, the network trained on them, then
     for n_test held-out Sobol points Newton to ||R|| < 1e-10 from the free
-    stream and from the learned guess.  Returns a JSON-able summary."""
+    stream and from the learned guess.  Returns a JSON-able summary;
+    progress(rows), when given, receives the held-out rows after each one."""
     t = dev.torch()
     params = sobol_params(n_train + n_test, seed=seed)
     train_p, test_p = params[:n_train], params[n_train:]
@@ -326,9 +380,8 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
     rows = []
     for re, ma in test_p:
         f = make_flow(im, jm, re, ma)
-        w0 = freestream(f)
-        free = interior_field(f, w0)
-        plain = solve(f, w0)
+        free = interior_field(f, freestream(f))
+        plain, w0 = solve_plain(f)
         t.cuda.synchronize()
         t1 = time.perf_counter()
         guess = guess_field(f, net, re, ma)       # inference is part of the learned start's time
@@ -338,16 +391,21 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
         sol = interior_field(f, wl)
         rows.append({
             "re": float(re), "mach": float(ma),
-            "plain": {"seconds": plain.seconds, "newton_steps": plain.iterations, "converged": plain.converged},
+            "plain": {"seconds": plain.seconds, "newton_steps": plain.iterations, "converged": plain.converged,
+                      "start": plain.start},
             "learned": {"seconds": learned.seconds + infer_s, "newton_steps": learned.iterations,
                         "converged": learned.converged, "start": learned.start, "inference_s": infer_s},
             "distance_to_baseflow": {"free_stream": float((free - sol).norm() / sol.norm()),
                                      "guess": float((guess - sol).norm() / sol.norm())},
             "same_baseflow": float((interior_field(f, w0) - sol).norm() / sol.norm()),
         })
-        log(f"held out Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps {plain.seconds:.2f} s, "
-            f"learned {learned.iterations} steps {learned.seconds + infer_s:.2f} s ({learned.start}), "
-            f"guess distance {rows[-1]['distance_to_baseflow']['guess']:.2e}")
+        log(f"held out Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps {plain.seconds:.2f} s "
+            f"({plain.start}), learned {learned.iterations} steps {learned.seconds + infer_s:.2f} s "
+            f"({learned.start}), guess distance {rows[-1]['distance_to_baseflow']['guess']:.2e}")
+        import json as _json
+        log("ROW " + _json.dumps(rows[-1]))
+        if progress is not None:
+            progress(rows)
     ps = np.array([r["plain"]["seconds"] for r in rows])
     ls = np.array([r["learned"]["seconds"] for r in rows])
     pi = np.array([r["plain"]["newton_steps"] for r in rows])
@@ -365,6 +423,8 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
         "baseflow_generation_s": gen_s, "training_s": train_s, "training_steps": steps,
         "loss_first_last": [hist[0], hist[-1]],
         "training_newton_steps_mean": float(np.mean([s.iterations for s in stats])),
+        "training_starts": "in order of Re, each from the nearest converged baseflow (continuation in Re as "
+                           "fallback)",
         "held_out_results": rows,
         "speedup_wall_converged": float(ps[both].sum() / ls[both].sum()) if both.any() else None,
         "converged_points": int(both.sum()), "not_converged": unconv,
@@ -393,7 +453,14 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    res = experiment(a.im, a.jm, a.train, a.test, a.width, a.modes, a.steps, log=lambda s: print(s, flush=True))
+    def progress(rows):
+        # the held-out rows so far, so that a run cut short keeps them
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump({"partial": True, "mesh": [a.im, a.jm], "held_out_results": rows}, fh, indent=1)
+
+    res = experiment(a.im, a.jm, a.train, a.test, a.width, a.modes, a.steps, log=lambda s: print(s, flush=True),
+                     progress=progress)
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

@@ -109,3 +109,36 @@ def test_learned_guess_beats_free_stream(torch_mod):
         assert float((L.interior_field(f, w0) - sol).norm() / sol.norm()) < 1e-8   # the same baseflow
         assert e_guess < 0.5 * e_free
         assert learned.iterations <= plain.iterations
+
+
+def test_learned_guess_at_config_size(torch_mod):
+    """The learned start on the configured O-mesh (384 x 256): 4 training
+    baseflows (warm-started in Re), one held-out point; the guess is closer
+    to the baseflow than the free stream, Newton from it converges to the
+    same baseflow in fewer steps than from the free stream."""
+    from broadcast_amd import learned as L
+
+    im, jm = 384, 256
+    params = L.sobol_params(5, seed=21)
+    train_p, test_p = params[:4], params[4:]
+    print(f"train {train_p.tolist()} held out {test_p.tolist()}", flush=True)
+    fields, stats, train_p = L.baseflows(im, jm, train_p, log=lambda s: print(s, flush=True))
+    assert len(train_p) == 4
+    net = L.GuessNet.create(fields, width=32, modes=4, seed=0)
+    L.train(net, train_p, fields, steps=600, lr=3e-3)
+    re, ma = test_p[0]
+    f = L.make_flow(im, jm, re, ma)
+    free = L.interior_field(f, L.freestream(f))
+    plain, w0 = L.solve_plain(f)
+    guess = L.guess_field(f, net, re, ma)
+    learned, wl = L.solve_from_guess(f, guess)
+    sol = L.interior_field(f, wl)
+    e_guess = float((guess - sol).norm() / sol.norm())
+    e_free = float((free - sol).norm() / sol.norm())
+    print(f"Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps {plain.seconds:.1f} s, learned "
+          f"{learned.iterations} steps {learned.seconds:.1f} s; distance: free stream {e_free:.2e}, "
+          f"guess {e_guess:.2e}", flush=True)
+    assert plain.converged and learned.converged and learned.start == "learned"
+    assert float((L.interior_field(f, w0) - sol).norm() / sol.norm()) < 1e-8
+    assert e_guess < 0.5 * e_free
+    assert learned.iterations < plain.iterations
