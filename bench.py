@@ -327,10 +327,10 @@ def main():
     print(f"bench: {ms_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f}) kernels {kernels}",
           file=sys.stderr, flush=True)
 
-    # Algorithmic bytes of the step (SURVEY 8(d) for the colored Jacobian):
-    # 8 B per CSR value written + 96 B per cell of shared inputs (w and
-    # metrics); the face blocks are scratch and the CSR indices a per-mesh
-    # pattern.  The roofline line gives these bytes to the dominant kernel.
+    # Algorithmic bytes of the step for the colored Jacobian: 8 B per CSR
+    # value written + 96 B per cell of shared inputs (w and metrics); the
+    # face blocks are scratch and the CSR indices a per-mesh pattern.  The
+    # roofline line gives these bytes to the dominant kernel.
     step_alg = 8 * nnz + JVP_SHARED_B_PER_CELL * cells
     nfi, nfj = (case.im + 1) * case.jm, case.im * (case.jm + 1)
     # bytes each kernel moves by design (face blocks are written once, read by
@@ -343,7 +343,7 @@ def main():
         "ghost_tangents": 0,
         "k_assemble_ghosts": 0,
     }
-    dom = max(names, key=lambda k: kernels[k]["total_ms"])
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     avg_dom = kernels[dom]["avg_ms"]
     ach_dom = step_alg / (avg_dom * 1e-3) / 1e9
     ach_dom_design = by_design[dom] / (avg_dom * 1e-3) / 1e9
