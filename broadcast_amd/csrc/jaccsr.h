@@ -171,17 +171,52 @@ __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T,
   double v[CSR_MASK_WORDS];
   unsigned long long mk[CSR_MASK_WORDS];
   bool oob = false;
+  // the stored masks and the row start, loaded ahead of the values
+  unsigned long long old[CSR_MASK_WORDS];
+  int pos0 = 0;
+  if constexpr (MODE == JR_REFRESH) {
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) old[w] = masks[w * nt + tr];
+  }
+  if constexpr (MODE != JR_COUNT) pos0 = indptr[row];
+  // the lane's slots (word w, zone n): window indices first, then every
+  // ghost-part load issued at once (from entry 0 where a slot has none, the
+  // branch on tn being row-uniform), so that they are in flight together
+  int kks[CSR_MASK_WORDS], ls[CSR_MASK_WORDS][NZ];
+  double gv[CSR_MASK_WORDS][NZ];
 #pragma unroll
   for (int w = 0; w < CSR_MASK_WORDS; ++w) {
     const int t = w * 64 + lane;
-    const int a = t / 35, b = (t / 5) % 7, m5 = t % 5;
-    const int kk = t < NA * 35 ? (int)T.inv_k[b] : -1;
+    const int a = t / 35, b = (t / 5) % 7;
+    kks[w] = t < NA * 35 ? (int)T.inv_k[b] : -1;
+#pragma unroll
+    for (int n = 0; n < NZ; ++n) {
+      ls[w][n] = kks[w] >= 0 ? (int)T.inv_l[r][n][a] : -1;
+      gv[w][n] = 0.0;
+    }
+  }
+  if (tn >= 0) {
+#pragma unroll
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+      const int m5 = (w * 64 + lane) % 5;
+#pragma unroll
+      for (int n = 0; n < NZ; ++n) {
+        const int l = ls[w][n];
+        const long gi = l >= 0 ? (tn * (NZ * 49) + n * 49 + l * 7 + kks[w]) * 25 + e * 5 + m5 : 0;
+        gv[w][n] = G[gi];
+      }
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < CSR_MASK_WORDS; ++w) {
+    const int m5 = (w * 64 + lane) % 5;
+    const int kk = kks[w];
     double acc = 0.0;
     bool keep = false;
     if (kk >= 0) {
 #pragma unroll
       for (int n = 0; n < NZ; ++n) {
-        const int l = T.inv_l[r][n][a];
+        const int l = ls[w][n];
         if (l < 0) continue;
         int dl = l - rot;
         dl += dl < -3 ? 7 : 0;
@@ -190,7 +225,7 @@ __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T,
         const int c = T.vjk[kk] * 5 + T.vil[r][l + 7 * n] * jm * 5 + m5;
         double y = (xz == n) ? J[((kk * 7 + dl + 3) * 5 + m5) * JR_PITCH + r] : 0.0;
         if (has_cd && row == c) y += cd;
-        if (tn >= 0) y += G[(tn * (NZ * 49) + n * 49 + l * 7 + kk) * 25 + e * 5 + m5];
+        if (tn >= 0) y += gv[w][n];
         if (fabs(y) > mini) {
           acc += y;
           keep = true;
@@ -205,10 +240,10 @@ __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T,
   bool same = true;
   if constexpr (MODE == JR_REFRESH) {
 #pragma unroll
-    for (int w = 0; w < CSR_MASK_WORDS; ++w) same = same && masks[w * nt + tr] == mk[w];
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) same = (old[w] == mk[w]) & same;
   }
   if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
-    int pos = indptr[row];
+    int pos = pos0;
 #pragma unroll
     for (int w = 0; w < CSR_MASK_WORDS; ++w) {
       if ((mk[w] >> lane) & 1ull) {
@@ -276,6 +311,14 @@ __device__ inline void jr_row_plain(const PlainLane& pl, const double* J, int r,
   const int lane = threadIdx.x % 64;
   double v[JR_PLAIN_WORDS];
   unsigned long long mk[JR_PLAIN_WORDS];
+  // the stored masks and the row start, loaded ahead of the values
+  unsigned long long old[JR_PLAIN_WORDS];
+  int pos0 = 0;
+  if constexpr (MODE == JR_REFRESH) {
+#pragma unroll
+    for (int w = 0; w < JR_PLAIN_WORDS; ++w) old[w] = masks[w * nt + tr];
+  }
+  if constexpr (MODE != JR_COUNT) pos0 = indptr[row];
 #pragma unroll
   for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
     double acc = 0.0;
@@ -303,10 +346,10 @@ __device__ inline void jr_row_plain(const PlainLane& pl, const double* J, int r,
   bool same = true;
   if constexpr (MODE == JR_REFRESH) {
 #pragma unroll
-    for (int w = 0; w < JR_PLAIN_WORDS; ++w) same = same && masks[w * nt + tr] == mk[w];
+    for (int w = 0; w < JR_PLAIN_WORDS; ++w) same = (old[w] == mk[w]) & same;
   }
   if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
-    int pos = indptr[row];
+    int pos = pos0;
 #pragma unroll
     for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
       if ((mk[w] >> lane) & 1ull) {
