@@ -479,6 +479,13 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
     if (!EDGE && !pl && i <= im && !edge_tile) *bad = 2;
   }
   __syncthreads();
+  {
+    // a tile with no row for this launch leaves (the decision is the same
+    // for every thread of the block)
+    bool any = false;
+    for (int r = 0; r < JC_TI; ++r) any = any || (i0 + r <= im && (EDGE ? !plain[r] : plain[r]));
+    if (!any) return;
+  }
   const PlainLane pln = plain_lane(j);
 
 #pragma unroll 1
@@ -488,7 +495,9 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
     // no block keep them all in flight, selects keep the sums exact
 #pragma unroll
     for (int it = 0; it < JR_NIT; ++it) {
-      if (vdst[it] < 0) continue;
+      // only the rows this launch turns into CSR rows (plain rows here, the
+      // others in the edge launch)
+      if (vdst[it] < 0 || (EDGE ? plain[vdst[it] % JR_PITCH] : !plain[vdst[it] % JR_PITCH])) continue;
       const int h0 = vofs[it][0] >= 0, h1 = vofs[it][1] >= 0, h2 = vofs[it][2] >= 0, h3 = vofs[it][3] >= 0;
       const double x0 = bi[h0 ? vofs[it][0] + e * 5 * nfi : 0];
       const double x1 = bi[h1 ? vofs[it][1] + e * 5 * nfi : 0];
