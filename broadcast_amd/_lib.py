@@ -55,6 +55,8 @@ _SIGS = {
     "bca_residual_dnc3_nowall": [_dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_jvp_dnc3": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_jvp_dnc3_nowall": [_dp, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_residual_dnc": [_dp, C.c_int, C.c_int, _dp, _dp, Metrics, C.POINTER(Phys)],
+    "bca_jvp_dnc": [_dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_vjp_dnc5": [_dp, _dp, _dp, _dp, Metrics, C.POINTER(Phys)],
     "bca_bc_inlet": [_dp, _dp, _dp, C.c_int, C.POINTER(C.c_int), _dp, C.c_int, _dp, _dp, C.c_double],
     "bca_bc_noref": [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.c_double],

@@ -202,11 +202,11 @@ Phys to_phys(const bca_phys* p) {
 }
 Metric to_metric(const bca_metrics& g) { return Metric{g.nx, g.ny, g.vol, g.volf}; }
 
-// gh = 3 for the 5th-order scheme and every Jacobian path, gh = 2 for the
-// 3rd-order scheme alone (the reference's gh = (order + 1) / 2,
-// BROADCAST_npz.py:501-502)
+// gh = 3 for the 5th-order scheme and every Jacobian path; the residual and
+// tangent of order k need gh >= (k + 1) / 2 (the reference's gh = (order +
+// 1) / 2, BROADCAST_npz.py:501-502)
 int check_mesh(const Mesh& m) {
-  if (m.gh != 3 && m.gh != 2) return fail(-1, "gh must be 3 (5th-order DNC) or 2 (3rd-order DNC)");
+  if (m.gh < 2 || m.gh > 5) return fail(-1, "gh must be 2..5 (the DNC orders 3, 5, 7, 9 take (order + 1) / 2)");
   if (m.im < 5 || m.jm < 5) return fail(-1, "im and jm must be >= 5 (wall stencil reaches j = 5)");
   return 0;
 }
@@ -230,8 +230,11 @@ int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g
   const dim3 b(64, 4);
   {
     Timed t(c, "k_grad" + sfx);
-    if (ord == 3) k_grad<T, 3><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
-    else k_grad<T><<<grid2(m.im, m.jm, b), b, 0, s>>>(m, g, wk.prim, wk.grad);
+    const dim3 gg = grid2(m.im, m.jm, b);
+    if (ord == 3) k_grad<T, 3><<<gg, b, 0, s>>>(m, g, wk.prim, wk.grad);
+    else if (ord == 7) k_grad<T, 7><<<gg, b, 0, s>>>(m, g, wk.prim, wk.grad);
+    else if (ord == 9) k_grad<T, 9><<<gg, b, 0, s>>>(m, g, wk.prim, wk.grad);
+    else k_grad<T><<<gg, b, 0, s>>>(m, g, wk.prim, wk.grad);
   }
   BCA_LAUNCHED();
   k_grad_ghost<T><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, wk.grad);
@@ -240,8 +243,14 @@ int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g
   {
     Timed t(c, "k_face" + sfx);
     const dim3 gf = grid2(m.im + 1, m.jm + 1, bf);
-    if (ord == 3 && nowall) k_face<T, true, 3><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
-    else if (ord == 3) k_face<T, false, 3><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
+#define BCA_FACE(NW, ORD) k_face<T, NW, ORD><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn)
+    if (ord == 3 && nowall) BCA_FACE(true, 3);
+    else if (ord == 3) BCA_FACE(false, 3);
+    else if (ord == 7 && nowall) BCA_FACE(true, 7);
+    else if (ord == 7) BCA_FACE(false, 7);
+    else if (ord == 9 && nowall) BCA_FACE(true, 9);
+    else if (ord == 9) BCA_FACE(false, 9);
+#undef BCA_FACE
     else if (nowall) k_face<T, true><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
     else k_face<T><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
   }
@@ -496,10 +505,23 @@ int bca_volf(bca_ctx* c, const double* vol, double* volf) {
   return 0;
 }
 
+// the scheme of order ord on the context's mesh: a known order, ghost
+// layers for its widest stencil, rows for its near-wall closures
+static int order_ok(const bca_ctx* c, int ord, bool nowall) {
+  if (ord != 3 && ord != 5 && ord != 7 && ord != 9) return fail(-1, "DNC order must be 3, 5, 7 or 9");
+  if (ord == 5) return need_gh3(c);
+  if (c->m.gh < (ord + 1) / 2)
+    return fail(-1, "the DNC order " + std::to_string(ord) + " needs gh >= " + std::to_string((ord + 1) / 2));
+  const int rows = ord == 7 ? 9 : ord == 9 ? 11 : 5;
+  if (!nowall && c->m.jm < rows)
+    return fail(-1, "the wall closure of DNC order " + std::to_string(ord) + " reaches j = " + std::to_string(rows));
+  return 0;
+}
+
 static int residual_impl(bca_ctx* c, double* residu, const double* w, bca_metrics g, const bca_phys* p,
                          bool nowall, int ord) {
   if (!c || !residu || !w || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
-  int rc = ord == 5 ? need_gh3(c) : 0;
+  int rc = order_ok(c, ord, nowall);
   if (rc) return rc;
   if ((rc = ensure_work(c, c->w0, false))) return rc;
   return run_pipeline<double>(c, c->w0, w, residu, to_metric(g), to_phys(p), nowall, ord);
@@ -591,7 +613,7 @@ int bca_residual_dnc5_noalloc(double* residu, const double* w, bca_metrics g, co
 static int jvp_impl(bca_ctx* c, double* residu, double* residud, const double* w, const double* wd,
                     bca_metrics g, const bca_phys* p, bool nowall, int ord) {
   if (!c || !w || !wd || !p || !g.nx || !g.ny || !g.vol || !g.volf) return fail(-1, "null argument");
-  int rc = ord == 5 ? need_gh3(c) : 0;
+  int rc = order_ok(c, ord, nowall);
   if (rc) return rc;
   if ((rc = ensure_work(c, c->w1, true))) return rc;
   const Mesh& m = c->m;
@@ -643,6 +665,18 @@ int bca_jvp_dnc3_nowall(bca_ctx* c, double* residu, double* residud, const doubl
   (void)residu;
   if (!residud) return fail(-1, "null argument");
   return jvp_impl(c, nullptr, residud, w, wd, g, p, true, 3);
+}
+
+int bca_residual_dnc(bca_ctx* c, int order, int nowall, double* residu, const double* w, bca_metrics g,
+                     const bca_phys* p) {
+  return residual_impl(c, residu, w, g, p, nowall != 0, order);
+}
+
+int bca_jvp_dnc(bca_ctx* c, int order, int nowall, double* residu, double* residud, const double* w,
+                const double* wd, bca_metrics g, const bca_phys* p) {
+  (void)residu;
+  if (!residud) return fail(-1, "null argument");
+  return jvp_impl(c, nullptr, residud, w, wd, g, p, nowall != 0, order);
 }
 
 static int bc_entry(bca_ctx* c, double* w, double* wd, const bca_bc_op& op, const double* nx,

@@ -380,7 +380,27 @@ __device__ inline auto iface_viscous(const A& a) {
 // predictor_7p_i.F), 4 points for ORD 3 (euler_o4_i.F, predictor_5p_i.F)
 template <int ORD, int DX, int DY, class A>
 __device__ inline auto euler_centred(const A& a, int e, double nxf, double nyf) {
-  if constexpr (ORD == 3)
+  if constexpr (ORD == 7)
+    return (CoefHi<7>::c(0) * (efx<0, 0>(a, e) + efx<-DX, -DY>(a, e)) +
+            CoefHi<7>::c(1) * (efx<DX, DY>(a, e) + efx<-2 * DX, -2 * DY>(a, e)) +
+            CoefHi<7>::c(2) * (efx<2 * DX, 2 * DY>(a, e) + efx<-3 * DX, -3 * DY>(a, e)) +
+            CoefHi<7>::c(3) * (efx<3 * DX, 3 * DY>(a, e) + efx<-4 * DX, -4 * DY>(a, e))) * nxf +
+           (CoefHi<7>::c(0) * (efy<0, 0>(a, e) + efy<-DX, -DY>(a, e)) +
+            CoefHi<7>::c(1) * (efy<DX, DY>(a, e) + efy<-2 * DX, -2 * DY>(a, e)) +
+            CoefHi<7>::c(2) * (efy<2 * DX, 2 * DY>(a, e) + efy<-3 * DX, -3 * DY>(a, e)) +
+            CoefHi<7>::c(3) * (efy<3 * DX, 3 * DY>(a, e) + efy<-4 * DX, -4 * DY>(a, e))) * nyf;
+  else if constexpr (ORD == 9)
+    return (CoefHi<9>::c(0) * (efx<0, 0>(a, e) + efx<-DX, -DY>(a, e)) +
+            CoefHi<9>::c(1) * (efx<DX, DY>(a, e) + efx<-2 * DX, -2 * DY>(a, e)) +
+            CoefHi<9>::c(2) * (efx<2 * DX, 2 * DY>(a, e) + efx<-3 * DX, -3 * DY>(a, e)) +
+            CoefHi<9>::c(3) * (efx<3 * DX, 3 * DY>(a, e) + efx<-4 * DX, -4 * DY>(a, e)) +
+            CoefHi<9>::c(4) * (efx<4 * DX, 4 * DY>(a, e) + efx<-5 * DX, -5 * DY>(a, e))) * nxf +
+           (CoefHi<9>::c(0) * (efy<0, 0>(a, e) + efy<-DX, -DY>(a, e)) +
+            CoefHi<9>::c(1) * (efy<DX, DY>(a, e) + efy<-2 * DX, -2 * DY>(a, e)) +
+            CoefHi<9>::c(2) * (efy<2 * DX, 2 * DY>(a, e) + efy<-3 * DX, -3 * DY>(a, e)) +
+            CoefHi<9>::c(3) * (efy<3 * DX, 3 * DY>(a, e) + efy<-4 * DX, -4 * DY>(a, e)) +
+            CoefHi<9>::c(4) * (efy<4 * DX, 4 * DY>(a, e) + efy<-5 * DX, -5 * DY>(a, e))) * nyf;
+  else if constexpr (ORD == 3)
     return (Coef3::c1 * (efx<0, 0>(a, e) + efx<-DX, -DY>(a, e)) +
             Coef3::c2 * (efx<DX, DY>(a, e) + efx<-2 * DX, -2 * DY>(a, e))) * nxf +
            (Coef3::c1 * (efy<0, 0>(a, e) + efy<-DX, -DY>(a, e)) +
@@ -395,7 +415,21 @@ __device__ inline auto euler_centred(const A& a, int e, double nxf, double nyf) 
 }
 template <int ORD, int DX, int DY, class A>
 __device__ inline auto predictor(const A& a, int e) {
-  if constexpr (ORD == 3)
+  // predictor_9p_i.F, predictor_11p_i.F: alternating signs from the
+  // outermost left cell to the outermost right one
+  if constexpr (ORD == 7)
+    return CoefHi<7>::d(3) * a.template W<-4 * DX, -4 * DY>(e) - CoefHi<7>::d(2) * a.template W<-3 * DX, -3 * DY>(e) +
+           CoefHi<7>::d(1) * a.template W<-2 * DX, -2 * DY>(e) - CoefHi<7>::d(0) * a.template W<-DX, -DY>(e) +
+           CoefHi<7>::d(0) * a.template W<0, 0>(e) - CoefHi<7>::d(1) * a.template W<DX, DY>(e) +
+           CoefHi<7>::d(2) * a.template W<2 * DX, 2 * DY>(e) - CoefHi<7>::d(3) * a.template W<3 * DX, 3 * DY>(e);
+  else if constexpr (ORD == 9)
+    return -(CoefHi<9>::d(4) * a.template W<-5 * DX, -5 * DY>(e)) +
+           CoefHi<9>::d(3) * a.template W<-4 * DX, -4 * DY>(e) - CoefHi<9>::d(2) * a.template W<-3 * DX, -3 * DY>(e) +
+           CoefHi<9>::d(1) * a.template W<-2 * DX, -2 * DY>(e) - CoefHi<9>::d(0) * a.template W<-DX, -DY>(e) +
+           CoefHi<9>::d(0) * a.template W<0, 0>(e) - CoefHi<9>::d(1) * a.template W<DX, DY>(e) +
+           CoefHi<9>::d(2) * a.template W<2 * DX, 2 * DY>(e) - CoefHi<9>::d(3) * a.template W<3 * DX, 3 * DY>(e) +
+           CoefHi<9>::d(4) * a.template W<4 * DX, 4 * DY>(e);
+  else if constexpr (ORD == 3)
     return Coef3::d2 * a.template W<-2 * DX, -2 * DY>(e) - Coef3::d1 * a.template W<-DX, -DY>(e) +
            Coef3::d1 * a.template W<0, 0>(e) - Coef3::d2 * a.template W<DX, DY>(e);
   else
@@ -504,10 +538,33 @@ __device__ inline auto jface_viscous(const A& a) {
 // the off-centred ones read cells of rows 1..5 (flux_num_dnc5.F90:164-196).
 // ORD 3 has one off-centred row, ROW 2, the mean of rows 1 and 2
 // (nearbndfluxes3demi_5p.F, flux_num_dnc3.F90:169-172).
+// sum_r w(ROW, r) f(cell row r + 1), r = 0 .. NW-1, added left to right
+// (nearbndfluxes*_9p.F, *_11p.F); R0 = 1 - ROW is cell row 1's offset from
+// the face row
+template <int R0, int R, int ORD, bool Y, class A, class S>
+__device__ inline auto wall_acc(const A& a, int e, S acc) {
+  if constexpr (R < CoefHi<ORD>::NW) {
+    constexpr int ROW = 1 - R0;
+    if constexpr (Y) return wall_acc<R0, R + 1, ORD, Y>(a, e, acc + CoefHi<ORD>::w(ROW, R) * efy<0, R0 + R>(a, e));
+    else return wall_acc<R0, R + 1, ORD, Y>(a, e, acc + CoefHi<ORD>::w(ROW, R) * efx<0, R0 + R>(a, e));
+  } else {
+    return acc;
+  }
+}
+template <int R0, int ORD, bool Y, class A>
+__device__ inline auto wall_sum(const A& a, int e) {
+  constexpr int ROW = 1 - R0;
+  if constexpr (Y) return wall_acc<R0, 1, ORD, Y>(a, e, CoefHi<ORD>::w(ROW, 0) * efy<0, R0>(a, e));
+  else return wall_acc<R0, 1, ORD, Y>(a, e, CoefHi<ORD>::w(ROW, 0) * efx<0, R0>(a, e));
+}
+
 template <int ROW, int ORD = 5, class A>
 __device__ inline auto jeuler(const A& a, int e, double n2x, double n2y) {
   if constexpr (ROW == 0) {
     return euler_centred<ORD, 0, 1>(a, e, n2x, n2y);
+  } else if constexpr (ORD == 7 || ORD == 9) {
+    // nearbndfluxes{3,5,7,9}demi_{9,11}p.F: cell rows 1 .. NW
+    return wall_sum<1 - ROW, ORD, false>(a, e) * n2x + wall_sum<1 - ROW, ORD, true>(a, e) * n2y;
   } else if constexpr (ORD == 3) {
     static_assert(ROW == 2, "the 3rd-order scheme has one off-centred row");
     return (0.5 * efx<0, -1>(a, e) + 0.5 * efx<0, 0>(a, e)) * n2x +
@@ -576,14 +633,29 @@ __device__ inline void jface_wall(const A& a, typename A::Out hn[5]) {
 template <int ORD = 5, class A>
 __device__ inline void iface_any(const A& a, typename A::Out hn[5]) {
   if constexpr (ORD == 3) iface<false, 3>(a, hn);
-  else if (a.j >= 3) iface<true>(a, hn);
-  else iface<false>(a, hn);
+  else if (a.j >= 3) iface<true, ORD>(a, hn);
+  else iface<false, ORD>(a, hn);
 }
 template <int ORD = 5, class A>
 __device__ inline void jface_any(const A& a, typename A::Out hn[5]) {
   if constexpr (ORD == 3) {
     if (a.j >= 3) jface<0, false, 3>(a, hn);
     else if (a.j == 2) jface<2, false, 3>(a, hn);
+    else jface_wall(a, hn);
+  } else if constexpr (ORD == 7) {
+    // flux_num_dnc7.F90:117-176: off-centred Euler rows 2 .. 4
+    if (a.j >= 5) jface<0, true, 7>(a, hn);
+    else if (a.j == 4) jface<4, true, 7>(a, hn);
+    else if (a.j == 3) jface<3, true, 7>(a, hn);
+    else if (a.j == 2) jface<2, false, 7>(a, hn);
+    else jface_wall(a, hn);
+  } else if constexpr (ORD == 9) {
+    // flux_num_dnc9.F90: off-centred Euler rows 2 .. 5
+    if (a.j >= 6) jface<0, true, 9>(a, hn);
+    else if (a.j == 5) jface<5, true, 9>(a, hn);
+    else if (a.j == 4) jface<4, true, 9>(a, hn);
+    else if (a.j == 3) jface<3, true, 9>(a, hn);
+    else if (a.j == 2) jface<2, false, 9>(a, hn);
     else jface_wall(a, hn);
   } else {
     if (a.j >= 4) jface<0, true>(a, hn);
@@ -598,8 +670,8 @@ __device__ inline void jface_any(const A& a, typename A::Out hn[5]) {
 // NOWALL: the variants without wall rows (srcfv/rhs/flux_num_dnc5_nowall.F90:
 // 140-154, flux_num_dnc3_nowall.F90): every face takes the interior formulas,
 // the centred Euler flux and the viscous flux of the order (compact 4th-order
-// for ORD 5, 2nd-order for ORD 3), the ghost layers supplying the stencil at
-// the block edges.  ORD: 5 = flux_num_dnc5*, 3 = flux_num_dnc3*.
+// for ORD 5, 7, 9, 2nd-order for ORD 3), the ghost layers supplying the stencil at
+// the block edges.  ORD: 3, 5, 7, 9 = flux_num_dnc3*, _dnc5*, _dnc7*, _dnc9*.
 template <class T, bool NOWALL = false, int ORD = 5>
 __global__ void __launch_bounds__(256) k_face(Mesh m, Phys ph, Metric g, const T* __restrict__ w,
                                               const T* __restrict__ prim, const T* __restrict__ grad,
@@ -613,13 +685,13 @@ __global__ void __launch_bounds__(256) k_face(Mesh m, Phys ph, Metric g, const T
   const long f = nidx(m, i, j);
   T h[5];
   if (j <= m.jm) {
-    if constexpr (NOWALL) iface<ORD == 5, ORD>(a, h);
+    if constexpr (NOWALL) iface<ORD != 3, ORD>(a, h);
     else iface_any<ORD>(a, h);
 #pragma unroll
     for (int e = 0; e < 5; ++e) hn[f + (long)e * m.nnode] = h[e];
   }
   if (i <= m.im) {
-    if constexpr (NOWALL) jface<0, ORD == 5, ORD>(a, h);
+    if constexpr (NOWALL) jface<0, ORD != 3, ORD>(a, h);
     else jface_any<ORD>(a, h);
 #pragma unroll
     for (int e = 0; e < 5; ++e) hn[f + (long)(5 + e) * m.nnode] = h[e];

@@ -57,6 +57,61 @@ struct Coef3 {
   static constexpr double c1 = 7.0 / 12.0, c2 = -1.0 / 12.0;
   static constexpr double d1 = 3.0 / 12.0, d2 = 1.0 / 12.0;
 };
+// Orders 7 and 9 (srcfv/rhs/flux_num_dnc7.F90:74-90, coefnearbnd_9p.F;
+// flux_num_dnc9.F90:106-128, coefnearbnd_11p.F), formed as the reference
+// forms them: b(n) the centred gradient, c(n) the centred Euler flux and d(n)
+// the predictor (n = 0 the innermost pair), w(row, r) the off-centred Euler
+// flux of the near-wall face row `row` on cell rows 1 .. NW.
+template <int ORD>
+struct CoefHi;
+template <>
+struct CoefHi<7> {
+  static constexpr int NB = 3, NC = 4, NW = 9, NROW = 3;
+  static constexpr double b(int n) {
+    return (n == 0 ? 45.0 : n == 1 ? -9.0 : 1.0) * (1.0 / 60.0);
+  }
+  static constexpr double c(int n) {
+    return (n == 0 ? 533.0 : n == 1 ? -139.0 : n == 2 ? 29.0 : -3.0) * (1.0 / 840.0);
+  }
+  static constexpr double d(int n) { return (n == 0 ? 35.0 : n == 1 ? 21.0 : n == 2 ? 7.0 : 1.0) * (1.0 / 280.0); }
+  static constexpr double w(int row, int r) {
+    constexpr double k3[9] = {92.0, 1547.0, -1861.0, 2171.0, -1917.0, 1191.0, -489.0, 119.0, -13.0};
+    constexpr double k5[9] = {-13.0, 209.0, 1079.0, -769.0, 533.0, -279.0, 99.0, -21.0, 2.0};
+    constexpr double k7[9] = {2.0, -31.0, 281.0, 911.0, -517.0, 281.0, -111.0, 27.0, -3.0};
+    return (row == 2 ? k3[r] : row == 3 ? k5[r] : k7[r]) * (1.0 / 840.0);
+  }
+};
+template <>
+struct CoefHi<9> {
+  static constexpr int NB = 4, NC = 5, NW = 11, NROW = 4;
+  static constexpr double b(int n) {
+    return (n == 0 ? 672.0 : n == 1 ? -168.0 : n == 2 ? 32.0 : -3.0) * (1.0 / 840.0);
+  }
+  static constexpr double c(int n) {
+    return (n == 0 ? 1627.0 : n == 1 ? -473.0 : n == 2 ? 127.0 : n == 3 ? -23.0 : 2.0) * (1.0 / 2520.0);
+  }
+  static constexpr double d(int n) {
+    return n == 4 ? 1.0 / 1260.0 : (n == 0 ? 126.0 : n == 1 ? 84.0 : n == 2 ? 36.0 : 9.0) * (1.0 / 1260.0);
+  }
+  // coefnearbnd_11p.F: the 9/2 face's coefficients from the centred ones
+  // minus a correction, each row below from the one above minus another
+  static constexpr double w(int row, int r) {
+    constexpr double a9[11] = {1.0 / 840.0, -1.0 / 63.0, 3.0 / 28.0, -4.0 / 7.0, -11.0 / 30.0, 6.0 / 5.0,
+                               -0.5,        4.0 / 21.0,  -3.0 / 56.0, 1.0 / 105.0, -1.0 / 1260.0};
+    constexpr double a7[11] = {-1.0 / 360.0, 1.0 / 24.0, -3.0 / 8.0, -319.0 / 420.0, 1.75, -21.0 / 20.0,
+                               7.0 / 12.0,   -0.25,      0.075,      -1.0 / 72.0,    1.0 / 840.0};
+    constexpr double a5[11] = {1.0 / 90.0, -2.0 / 9.0,  -341.0 / 280.0, 8.0 / 3.0,  -7.0 / 3.0, 28.0 / 15.0,
+                               -7.0 / 6.0, 8.0 / 15.0,  -1.0 / 6.0,     2.0 / 63.0, -1.0 / 360.0};
+    constexpr double a3[11] = {-0.1, -4609.0 / 2520.0, 4.5, -6.0, 7.0, -6.3, 21.0 * 0.2, -2.0, 9.0 / 14.0,
+                               -0.125, 1.0 / 90.0};
+    const double cc = r == 0 ? c(4) : r == 1 ? c(3) : r == 2 ? c(2) : r == 3 ? c(1) : r == 4 ? c(0)
+                    : r == 5 ? c(0) : r == 6 ? c(1) : r == 7 ? c(2) : r == 8 ? c(3) : r == 9 ? c(4) : 0.0;
+    const double w9 = r == 10 ? -a9[10] : cc - a9[r];
+    const double w7 = w9 - a7[r], w5 = w7 - a5[r], w3 = w5 - a3[r];
+    return row == 5 ? w9 : row == 4 ? w7 : row == 3 ? w5 : w3;
+  }
+};
+
 struct Coef {
   static constexpr double b1 = 8.0 / 12.0, b2 = -1.0 / 12.0;            // 4th-order gradient
   static constexpr double c1 = 37.0 / 60.0, c2 = -8.0 / 60.0, c3 = 1.0 / 60.0;  // centred Euler
@@ -112,7 +167,8 @@ __global__ void k_prim(Mesh m, Phys ph, const T* __restrict__ w, T* __restrict__
 }
 
 // ---------------------------------------------------------------- gradients
-// gradop_5pi/j.F (ORD 5) or gradop_3pi/j.F (ORD 3) + gradient.F + geom/dxdy.F
+// gradop_5pi/j.F (ORD 5), gradop_3pi/j.F (ORD 3), gradop_7pi/j.F, gradop_9pi/j.F
+// + gradient.F + geom/dxdy.F
 template <int ORD = 5, class T>
 __device__ inline void gradient_at(const Mesh& m, const Metric& g, const T* prim, int i, int j,
                                    T out[4]) {
@@ -121,15 +177,29 @@ __device__ inline void gradient_at(const Mesh& m, const Metric& g, const T* prim
   const T* V = prim + P_V * n;
   auto di = [&](const T* f) -> T {
     if constexpr (ORD == 3) return Coef3::b1 * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]);
-    else
+    else if constexpr (ORD == 5)
       return Coef::b1 * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]) +
              Coef::b2 * (f[cidx(m, i + 2, j)] - f[cidx(m, i - 2, j)]);
+    else {
+      T acc = CoefHi<ORD>::b(0) * (f[cidx(m, i + 1, j)] - f[cidx(m, i - 1, j)]);
+#pragma unroll
+      for (int n = 1; n < CoefHi<ORD>::NB; ++n)
+        acc = acc + CoefHi<ORD>::b(n) * (f[cidx(m, i + n + 1, j)] - f[cidx(m, i - n - 1, j)]);
+      return acc;
+    }
   };
   auto dj = [&](const T* f) -> T {
     if constexpr (ORD == 3) return Coef3::b1 * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]);
-    else
+    else if constexpr (ORD == 5)
       return Coef::b1 * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]) +
              Coef::b2 * (f[cidx(m, i, j + 2)] - f[cidx(m, i, j - 2)]);
+    else {
+      T acc = CoefHi<ORD>::b(0) * (f[cidx(m, i, j + 1)] - f[cidx(m, i, j - 1)]);
+#pragma unroll
+      for (int n = 1; n < CoefHi<ORD>::NB; ++n)
+        acc = acc + CoefHi<ORD>::b(n) * (f[cidx(m, i, j + n + 1)] - f[cidx(m, i, j - n - 1)]);
+      return acc;
+    }
   };
   T gui = di(U), gvi = di(V), guj = dj(U), gvj = dj(V);
   const long nn = m.nnode;

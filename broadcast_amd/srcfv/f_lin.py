@@ -46,6 +46,43 @@ def flux_num_dnc3_nowall_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, vol
              (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
 
 
+def _tangent_order(order, nowall, name, resd, w, wd, nx, ny, vol, volf, gh, phys, im, jm):
+    ctx = dev.context(im, jm, gh)
+    st = dev.stage(resd, w, wd, nx, ny, vol, volf, out=(0,))
+    g = _lib.Metrics(st[3].ptr, st[4].ptr, st[5].ptr, st[6].ptr)
+    ph = dev.phys(*phys)
+    _lib.check(_lib.lib().bca_jvp_dnc(ctx.h, order, int(nowall), None, st[0].ptr, st[1].ptr, st[2].ptr, g, ph), name)
+    dev.finish(st)
+
+
+def flux_num_dnc7_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,
+                       rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc7_d.f90, on arrays of ghost width gh = 4"""
+    _tangent_order(7, False, "flux_num_dnc7_2d_d", resd, w, wd, nx, ny, vol, volf, gh,
+                   (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc7_nowall_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl,
+                              gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc7_nowall_d.f90, on arrays of ghost width gh = 4"""
+    _tangent_order(7, True, "flux_num_dnc7_nowall_2d_d", resd, w, wd, nx, ny, vol, volf, gh,
+                   (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc9_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl, gam,
+                       rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc9_d.f90, on arrays of ghost width gh = 5"""
+    _tangent_order(9, False, "flux_num_dnc9_2d_d", resd, w, wd, nx, ny, vol, volf, gh,
+                   (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
+def flux_num_dnc9_nowall_2d_d(res, resd, w, wd, x0, y0, nx, ny, xc, yc, vol, volf, gh, cp, cv, prandtl,
+                              gam, rgaz, cs, muref, tref, s_suth, k2, k4, im, jm):
+    """srcfv/tangent/flux_num_dnc9_nowall_d.f90, on arrays of ghost width gh = 5"""
+    _tangent_order(9, True, "flux_num_dnc9_nowall_2d_d", resd, w, wd, nx, ny, vol, volf, gh,
+                   (cp, cv, prandtl, gam, rgaz, cs, muref, tref, s_suth, k2, k4), im, jm)
+
+
 def bc_supandsubinlet_2d_d(w, wd0, loc, interf, field, nx, ny, gam, im, jm, lm=None, gh=None):
     """srcfv/tangent/bc_supandsubinlet_d.f90:11"""
     _bc.inlet(w, wd0, loc, interf, field, nx, ny, gam, im, jm, lm, gh)

@@ -151,6 +151,21 @@ int bca_jvp_dnc3(bca_ctx *ctx, double *residu, double *residud, const double *w,
 int bca_jvp_dnc3_nowall(bca_ctx *ctx, double *residu, double *residud, const double *w,
                         const double *wd, bca_metrics g, const bca_phys *phys);
 
+/* Any order of the family with or without wall rows, on a context of
+ * gh >= (order + 1) / 2 (gh = 3 for order 5):
+ * flux_num_dnc{order}[_nowall]_2d(residu,w,...) replaces
+ * srcfv/rhs/flux_num_dnc{3,5,7,9}[_nowall].F90 (order 7: flux_num_dnc7.F90:
+ * 74-176, 8-point centred Euler flux, 9-point predictor, 6th-order gradients,
+ * off-centred Euler rows 2..4 (coefnearbnd_9p.F); order 9: flux_num_dnc9.F90,
+ * 10-point Euler, 11-point predictor, 8th-order gradients, rows 2..5
+ * (coefnearbnd_11p.F)); the wall variants need jm >= 9 (order 7) or 11
+ * (order 9).  bca_jvp_dnc replaces the matching
+ * srcfv/tangent/flux_num_dnc*_d.f90: residud only, residu left as given. */
+int bca_residual_dnc(bca_ctx *ctx, int order, int nowall, double *residu, const double *w, bca_metrics g,
+                     const bca_phys *phys);
+int bca_jvp_dnc(bca_ctx *ctx, int order, int nowall, double *residu, double *residud, const double *w,
+                const double *wd, bca_metrics g, const bca_phys *phys);
+
 /* ---- pruned CSR of the colored COO ---------------------------------------- */
 /* remove_zero_jac (BROADCAST_npz.py:129-135) + scipy.sparse.csr_matrix of
  * createMatPetscCSR (misc/PETSc_func.py:85) on a COO filled by the colored

@@ -122,6 +122,17 @@ class Oracle:
         args = [_p(a) for a in arrays] + [*self._i(gh), *self._r(*phys), *self._i(im, jm)]
         _run_big_stack(lambda: getattr(self.lib, name + "_")(*args))
 
+    def scheme(self, order, nowall, tangent):
+        """The reference's flux_num_dnc{order}[_nowall]_2d[_d] as a callable
+        with the f2py argument order (srcfv/rhs, srcfv/tangent)."""
+        name = f"flux_num_dnc{order}{'_nowall' if nowall else ''}_2d{'_d' if tangent else ''}"
+        narr = 12 if tangent else 10
+
+        def call(*args):
+            arrays, gh, rest = args[:narr], args[narr], args[narr + 1:]
+            self._scheme(name, arrays, gh, rest[:11], *rest[11:])
+        return call
+
     def flux_num_dnc3_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
         """srcfv/rhs/flux_num_dnc3.F90:7"""
         self._scheme("flux_num_dnc3_2d", (res, w, x0, y0, nx, ny, xc, yc, vol, volf), gh,
@@ -287,6 +298,17 @@ class Reference:
     def _scheme(self, name, arrays, gh, phys, im, jm):
         args = [_p(a) for a in arrays] + [*self._i(gh), *self._r(*phys), *self._i(im, jm)]
         _run_big_stack(lambda: getattr(self.lib, name + "_")(*args))
+
+    def scheme(self, order, nowall, tangent):
+        """The reference's flux_num_dnc{order}[_nowall]_2d[_d] as a callable
+        with the f2py argument order (srcfv/rhs, srcfv/tangent)."""
+        name = f"flux_num_dnc{order}{'_nowall' if nowall else ''}_2d{'_d' if tangent else ''}"
+        narr = 12 if tangent else 10
+
+        def call(*args):
+            arrays, gh, rest = args[:narr], args[narr], args[narr + 1:]
+            self._scheme(name, arrays, gh, rest[:11], *rest[11:])
+        return call
 
     def flux_num_dnc3_2d(self, res, w, x0, y0, nx, ny, xc, yc, vol, volf, gh, *phys_im_jm):
         """srcfv/rhs/flux_num_dnc3.F90:7"""

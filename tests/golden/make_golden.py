@@ -13,7 +13,9 @@ fixtures from their stored inputs without regenerating anything else;
 `--bcb-only` likewise adds the boundary-condition adjoints (bc_*_b),
 `--sbd-only` the outputs of set_bndbl_2d and `--nowall-only` the residual and
 tangent of the variant without wall rows (flux_num_dnc5_nowall_2d[_d]);
-`--dnc3-only` writes the 3rd-order scheme's fixtures (dnc3_*.npz, gh = 2).
+`--dnc3-only` writes the 3rd-order scheme's fixtures (dnc3_*.npz, gh = 2),
+`--dnc79-only` those of orders 7 and 9 (dnc7_*.npz, gh = 4; dnc9_*.npz,
+gh = 5).
 
 Fixtures (tests/golden/*.npz) hold inputs and expected outputs only.
 """
@@ -162,12 +164,14 @@ def nowall_vectors(R, fx):
     return dict(res_nowall=res, resd_nowall=resd)
 
 
-def dnc3_fixture(R, case, seed):
-    """The 3rd-order scheme on a gh = 2 case (BROADCAST_npz.py:501-502):
-    flux_num_dnc3_2d and flux_num_dnc3_nowall_2d (srcfv/rhs) on the state after
-    the case's BCs, and their tangents (srcfv/tangent/flux_num_dnc3*_d.f90)
-    along one seeded direction after the linearised BCs."""
-    assert case.gh == 2
+def order_fixture(R, case, seed, order):
+    """The scheme of another order on a case of gh = (order + 1) / 2
+    (BROADCAST_npz.py:501-502): flux_num_dnc{order}_2d and
+    flux_num_dnc{order}_nowall_2d (srcfv/rhs) on the state after the case's
+    BCs, and their tangents (srcfv/tangent/flux_num_dnc{order}*_d.f90) along
+    one seeded direction after the linearised BCs: res{order}[_nowall],
+    resd{order}[_nowall]."""
+    assert case.gh == (order + 1) // 2
     g = D.geometry(R, case)
     w = np.array(case.w, order="F", copy=True)
     D.apply_bcs(R, case, g, w)
@@ -180,8 +184,8 @@ def dnc3_fixture(R, case, seed):
     D.apply_bcs(R, case, g, wcopy, wd=d)
     args = case.phys.scheme_args()
     out = {}
-    for tag, fn, fnd in (("", R.flux_num_dnc3_2d, R.flux_num_dnc3_2d_d),
-                         ("_nowall", R.flux_num_dnc3_nowall_2d, R.flux_num_dnc3_nowall_2d_d)):
+    for tag, nowall in (("", False), ("_nowall", True)):
+        fn, fnd = R.scheme(order, nowall, False), R.scheme(order, nowall, True)
         res = np.zeros_like(w, order="F")
         fn(res, np.array(w, order="F", copy=True), g["x0"], g["y0"], g["nx"], g["ny"], g["xc"], g["yc"],
            g["vol"], g["volf"], gh, *args, im, jm)
@@ -190,8 +194,8 @@ def dnc3_fixture(R, case, seed):
         fnd(r2, resd, np.array(w, order="F", copy=True), np.array(d, order="F", copy=True), g["x0"], g["y0"],
             g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], gh, *args, im, jm)
         assert not r2.any()  # the generated tangent assigns residud only
-        out[f"res3{tag}"] = res
-        out[f"resd3{tag}"] = resd
+        out[f"res{order}{tag}"] = res
+        out[f"resd{order}{tag}"] = resd
     return dict(kind=np.array(case.kind), im=im, jm=jm, gh=gh, phys=np.array(args),
                 x0_in=case.x0, y0_in=case.y0, w_in=case.w,
                 field=case.field if case.field is not None else np.zeros((0,)),
@@ -220,8 +224,15 @@ CASES = ("bl24x16", "bl13x9", "cyl28x12")
 
 
 def DNC3_CASES(R):
-    return {"dnc3_bl20x12": dnc3_fixture(R, bl_case(20, 12, seed=6, gh=2), 16),
-            "dnc3_cyl21x10": dnc3_fixture(R, cylinder_case(21, 10, seed=7, gh=2), 17)}
+    return {"dnc3_bl20x12": order_fixture(R, bl_case(20, 12, seed=6, gh=2), 16, 3),
+            "dnc3_cyl21x10": order_fixture(R, cylinder_case(21, 10, seed=7, gh=2), 17, 3)}
+
+
+def DNC79_CASES(R):
+    return {"dnc7_bl22x14": order_fixture(R, bl_case(22, 14, seed=8, gh=4), 18, 7),
+            "dnc7_cyl24x12": order_fixture(R, cylinder_case(24, 12, seed=9, gh=4), 19, 7),
+            "dnc9_bl23x16": order_fixture(R, bl_case(23, 16, seed=10, gh=5), 20, 9),
+            "dnc9_cyl26x13": order_fixture(R, cylinder_case(26, 13, seed=11, gh=5), 21, 9)}
 
 
 def main():
@@ -229,8 +240,9 @@ def main():
     only = {"--dz-only": dz_vectors, "--bcb-only": bcb_vectors, "--sbd-only": sbd_vectors,
             "--nowall-only": nowall_vectors}
     extra = [fn for flag, fn in only.items() if flag in sys.argv]
-    if "--dnc3-only" in sys.argv:
-        for name, fx in DNC3_CASES(R).items():
+    if "--dnc3-only" in sys.argv or "--dnc79-only" in sys.argv:
+        cases = DNC3_CASES(R) if "--dnc3-only" in sys.argv else DNC79_CASES(R)
+        for name, fx in cases.items():
             path = os.path.join(OUT, f"{name}.npz")
             np.savez_compressed(path, **fx)
             print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
@@ -257,6 +269,7 @@ def main():
         fixtures[name].update(sbd_vectors(R, fixtures[name]))
         fixtures[name].update(nowall_vectors(R, fixtures[name]))
     fixtures.update(DNC3_CASES(R))
+    fixtures.update(DNC79_CASES(R))
     for name, fx in fixtures.items():
         path = os.path.join(OUT, f"{name}.npz")
         np.savez_compressed(path, **fx)

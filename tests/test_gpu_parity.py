@@ -178,18 +178,20 @@ def test_nowall_variant_matches_reference(torch_mod, name):
     assert (res2 == 7.0).all()
 
 
-@pytest.mark.parametrize("name", ["dnc3_bl20x12", "dnc3_cyl21x10"])
-def test_dnc3_schemes_match_reference(torch_mod, name):
-    """The 3rd-order scheme on gh = 2 arrays, through the drop-in call forms:
-    geometry, the case's BCs and their linearisation, flux_num_dnc3_2d and
-    flux_num_dnc3_nowall_2d (srcfv/rhs) and their tangents (srcfv/tangent/
-    flux_num_dnc3*_d.f90) against the reference's outputs."""
+@pytest.mark.parametrize("name,order", [("dnc3_bl20x12", 3), ("dnc3_cyl21x10", 3), ("dnc7_bl22x14", 7),
+                                        ("dnc7_cyl24x12", 7), ("dnc9_bl23x16", 9), ("dnc9_cyl26x13", 9)])
+def test_other_orders_match_reference(torch_mod, name, order):
+    """The schemes of order 3, 7 and 9 on arrays of gh = (order + 1) / 2,
+    through the drop-in call forms: geometry, the case's BCs and their
+    linearisation, flux_num_dnc{order}_2d and flux_num_dnc{order}_nowall_2d
+    (srcfv/rhs) and their tangents (srcfv/tangent/flux_num_dnc{order}*_d.f90)
+    against the reference's outputs."""
     from broadcast_amd.srcfv import f_bnd, f_geom, f_lin, f_sch
 
     fx = load(name)
     case = case_from(fx)
     im, jm, gh = case.im, case.jm, case.gh
-    assert gh == 2
+    assert gh == (order + 1) // 2
     g = {k: np.zeros_like(fx[f"geo_{k}"], order="F") for k in GEO}
     x0, y0 = np.array(fx["x0_in"], order="F"), np.array(fx["y0_in"], order="F")
     f_geom.computegeom_2d(x0, y0, g["nx"], g["ny"], g["xc"], g["yc"], g["vol"], g["volf"], im, jm, gh)
@@ -232,16 +234,17 @@ def test_dnc3_schemes_match_reference(torch_mod, name):
     common = (geo["x0"], geo["y0"], geo["nx"], geo["ny"], geo["xc"], geo["yc"], geo["vol"], geo["volf"], gh,
               *args, im, jm)
     sl = (slice(gh, -gh), slice(gh, -gh))
-    for tag, fn, fnd in (("", f_sch.flux_num_dnc3_2d, f_lin.flux_num_dnc3_2d_d),
-                         ("_nowall", f_sch.flux_num_dnc3_nowall_2d, f_lin.flux_num_dnc3_nowall_2d_d)):
+    for tag in ("", "_nowall"):
+        fn = getattr(f_sch, f"flux_num_dnc{order}{tag}_2d")
+        fnd = getattr(f_lin, f"flux_num_dnc{order}{tag}_2d_d")
         res = np.zeros_like(w, order="F")
         fn(res, w, *common)
-        ok, err = close(res[sl], fx[f"res3{tag}"][sl], 1e-12)
+        ok, err = close(res[sl], fx[f"res{order}{tag}"][sl], 1e-12)
         assert ok, (tag, err)
         res2 = np.full_like(w, 7.0, order="F")
         resd = np.zeros_like(w, order="F")
         fnd(res2, resd, w, np.array(fx["wd_bc"], order="F"), *common)
-        ok, err = close(resd[sl], fx[f"resd3{tag}"][sl], 1e-12)
+        ok, err = close(resd[sl], fx[f"resd{order}{tag}"][sl], 1e-12)
         assert ok, (tag, err)
         assert (res2 == 7.0).all()
 
