@@ -1662,19 +1662,32 @@ int bca_inv_np(void* stream, double* A, int n, long ld, double* work, double tin
   const dim3 g((n + GJ_T - 1) / GJ_T, (n + GJ_T - 1) / GJ_T);
   double* buf[2] = {A, work};
   const long ldb[2] = {ld, (long)n};
-  int cur = 0;
-  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
-    const int nb = std::min(GJ_NB, n - k0);
+  // Panels of GJ_NB columns; the steps alternate between A and work, so an
+  // even number of them ends in A.  An odd count splits its last panel in
+  // two (or, when that panel is one column wide, the full panel before it),
+  // and n = 1 adds a step over no column (nb = 0 copies Ain to Aout).
+  std::vector<int> widths;
+  for (int k0 = 0; k0 < n; k0 += GJ_NB) widths.push_back(std::min(GJ_NB, n - k0));
+  if (widths.size() % 2 == 1) {
+    const int r = widths.back();
+    if (r >= 2) {
+      widths.back() = r - r / 2;
+      widths.push_back(r / 2);
+    } else if (widths.size() >= 2) {
+      widths[widths.size() - 2] = GJ_NB / 2;
+      widths.insert(widths.end() - 1, GJ_NB - GJ_NB / 2);
+    } else {
+      widths.push_back(0);
+    }
+  }
+  int cur = 0, k0 = 0;
+  for (const int nb : widths) {
     k_gj_step<<<g, 256, 0, s>>>(buf[cur], ldb[cur], buf[1 - cur], ldb[1 - cur], n, k0, nb, tiny, bad);
     BCA_LAUNCHED();
     cur = 1 - cur;
+    k0 += nb;
   }
-  if (cur == 1) {
-    // an odd number of steps leaves the inverse in work
-    if (hipMemcpy2DAsync(A, ld * sizeof(double), work, (size_t)n * sizeof(double), (size_t)n * sizeof(double), n,
-                         hipMemcpyDeviceToDevice, s) != hipSuccess)
-      return fail(-3, "bca_inv_np: copy back failed");
-  }
+  if (cur != 0 || k0 != n) return fail(-3, "bca_inv_np: internal panel schedule");
   return 0;
 }
 

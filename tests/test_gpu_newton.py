@@ -202,13 +202,15 @@ def test_lu_np_reproduces_the_block(torch_mod, n):
     assert int(bad.item()) == 1
 
 
-@pytest.mark.parametrize("n", [37, 200, 1001])
+@pytest.mark.parametrize("n", [37, 65, 200, 1001])
 def test_inv_np_inverts_the_block(torch_mod, n):
     """bca_inv_np (Gauss-Jordan without pivoting, ping-pong through work) on a
     block of the Newton matrices' kind inside a wider array: A^-1 A = I to
     1e-13 normwise, torch's pivoted inverse agrees, the columns outside the
-    block stay untouched; a zero leading pivot is reported.  n = 37 takes two
-    panel steps, 200 seven (the result comes back from work), 1001 32."""
+    block stay untouched; a zero leading pivot is reported.  The steps
+    alternate between the block and work, always an even number of them:
+    n = 37 in panels of 32 + 5, 65 in 32 + 16 + 16 + 1, 200 in six of 32 +
+    4 + 4, 1001 in 32 panels."""
     import ctypes as C
     from broadcast_amd import _lib
 
