@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 512 MiB device-to-device copy first (counter calibration under rocprofv3)")
+    ap.add_argument("--figure", choices=("residual_jvp", "vjp"), default=None,
+                    help="run one side figure alone and print its JSON (for counter runs under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures the single-GPU path only: every box here has one MI355X; "
@@ -214,6 +216,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.figure is not None:
+        fig = residual_jvp_figure() if args.figure == "residual_jvp" else vjp_figure()
+        print(json.dumps({"figure": args.figure, **fig}), flush=True)
+        return
     case = cylinder_case(args.im, args.jm, seed=1000 * 1 + 0)
     cells = case.im * case.jm
     f = Flow(case)

@@ -139,8 +139,10 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
                                                       const double* __restrict__ grad,
                                                       double* __restrict__ out, const double* __restrict__ resb,
                                                       int j0, int j1) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
+  int bx, by;
+  xcd_block(bx, by);
+  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int j = j0 + by * blockDim.y + threadIdx.y;
   if (i > m.im + 1 || j > j1) return;
   const long nf = (long)(m.im + 1) * m.jm;
   FaceSink<MODE> sink{out, nf, (i - 1) + (long)(m.im + 1) * (j - 1), {}};
@@ -167,8 +169,10 @@ __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g,
                                                       const double* __restrict__ grad,
                                                       double* __restrict__ out, const double* __restrict__ resb,
                                                       int j0, int j1) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
+  int bx, by;
+  xcd_block(bx, by);
+  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int j = j0 + by * blockDim.y + threadIdx.y;
   if (i > m.im || j > j1) return;
   const long nf = (long)m.im * (m.jm + 1);
   FaceSink<MODE> sink{out, nf, (i - 1) + (long)m.im * (j - 1), {}};

@@ -676,8 +676,10 @@ template <class T, bool NOWALL = false, int ORD = 5>
 __global__ void __launch_bounds__(256) k_face(Mesh m, Phys ph, Metric g, const T* __restrict__ w,
                                               const T* __restrict__ prim, const T* __restrict__ grad,
                                               T* __restrict__ hn) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  int bx, by;
+  xcd_block(bx, by);
+  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int j = 1 + by * blockDim.y + threadIdx.y;
   if (i > m.im + 1 || j > m.jm + 1) return;
   PlainAcc<T> a;
   a.m = m; a.ph = ph; a.g = g; a.i = i; a.j = j;

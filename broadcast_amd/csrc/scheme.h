@@ -33,6 +33,24 @@ __host__ inline Mesh make_mesh(int im, int jm, int gh) {
   return m;
 }
 
+// XCD-aware block order for the stencil kernels' 2-D grids.  MI355X hands
+// workgroups to its 8 XCDs in turn (workgroup b to XCD b % 8), each XCD with
+// its own L2; in launch order, blocks that share stencil cells (the rows
+// above and below) land on different XCDs and every L2 fetches its own
+// copy.  This gives XCD x one contiguous run of blocks in row-major order,
+// so the rows a block shares with its neighbours were fetched into the same
+// L2 a moment before.  Returns the logical (bx, by) of this workgroup.
+constexpr int NXCD = 8;
+__device__ inline void xcd_block(int& bx, int& by) {
+  const int nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  const int b = blockIdx.x + blockIdx.y * nbx;
+  const int q = nblk / NXCD, r = nblk % NXCD;
+  const int x = b % NXCD, s = b / NXCD;
+  const int t = x * q + (x < r ? x : r) + s;
+  bx = t % nbx;
+  by = t / nbx;
+}
+
 // Fortran (i, j), lower bound 1-gh, to the flat storage offset
 __host__ __device__ inline long cidx(const Mesh& m, int i, int j) {
   return (long)(i + m.gh - 1) + (long)m.ni * (j + m.gh - 1);
