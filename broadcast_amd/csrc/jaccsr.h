@@ -155,15 +155,69 @@ __host__ __device__ constexpr int jr_cell_dj(int c) {
 constexpr int JR_NELEM = JC_TI * JR_NCELL * 5;                   // (il, m, cell) value items of one e
 constexpr int JR_NIT = (JR_NELEM + JR_THREADS - 1) / JR_THREADS;
 
+// The row phase's loads of a wave's rows -- the stored mask words and the
+// row start of each variable row, coefdiag of each row cell -- issued once
+// before the variable loop, one item per lane (items past 64 in a second
+// register), and read back with readlane: the row phase then waits on no
+// global load.  Item q: row slot qr = q / PER_ROW, then per variable e the
+// W mask words (refresh) and indptr (fill, refresh), last coefdiag.
+__device__ inline unsigned long long readlane64(unsigned long long x, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+template <int MODE, int W, int RPW>
+struct RowPrefetch {
+  static constexpr int NMASK = MODE == JR_REFRESH ? W : 0;
+  static constexpr int PER_E = NMASK + (MODE != JR_COUNT ? 1 : 0);
+  static constexpr int PER_ROW = 5 * PER_E + 1;
+  static constexpr int NREG = (RPW * PER_ROW + 63) / 64;
+  unsigned long long v[NREG];
+  __device__ inline void load(const Mesh& m, int i0, int j, int wave, int nwave,
+                              const unsigned long long* masks, const int* indptr, const double* coefdiag) {
+    const int lane = threadIdx.x % 64;
+    const long nt = 5L * m.im * m.jm;
+#pragma unroll
+    for (int g = 0; g < NREG; ++g) {
+      const int q = g * 64 + lane;
+      const int qr = q / PER_ROW, x = q % PER_ROW;
+      const int i = i0 + wave + nwave * qr;
+      unsigned long long val = 0ull;
+      if (qr < RPW && i <= m.im) {
+        if (x == PER_ROW - 1) {
+          if (coefdiag) val = (unsigned long long)__double_as_longlong(coefdiag[(i - 1) + (long)m.im * (j - 1)]);
+        } else {
+          constexpr int DIV = PER_E > 0 ? PER_E : 1;  // PER_E = 0 (count) leaves only coefdiag
+          const int e = x / DIV, w = x % DIV;
+          if (w < NMASK)
+            val = masks[w * nt + csr_tid(m, i, j, e)];
+          else
+            val = (unsigned long long)(unsigned)indptr[e + (j - 1) * 5 + (i - 1) * m.jm * 5];
+        }
+      }
+      v[g] = val;
+    }
+  }
+  __device__ inline unsigned long long get(int q) const {
+    unsigned long long x = v[0];
+#pragma unroll
+    for (int g = 1; g < NREG; ++g) x = q >= g * 64 ? v[g] : x;
+    return readlane64(x, q % 64);
+  }
+  __device__ inline unsigned long long mask(int qr, int e, int w) const { return get(qr * PER_ROW + e * PER_E + w); }
+  __device__ inline int start(int qr, int e) const { return (int)get(qr * PER_ROW + e * PER_E + NMASK); }
+  __device__ inline double cd(int qr) const { return __longlong_as_double((long long)get(qr * PER_ROW + PER_ROW - 1)); }
+};
+
 // A row whose window is not the plain 7 x 7 neighbourhood (near the mesh
 // edge, where the ghost part is added from G when tn >= 0, or near a zone's
 // wrapped columns), in the edge launch: every slot from the tables, all
 // mask words in flight at once (this launch has a register budget of its
 // own).
-template <int NZ, int MODE>
+template <int NZ, int MODE, class PF>
 __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T, const Zones& z, int r, int i,
-                                   int jm, int row, long tr, long nt, int e, double cd, bool has_cd, const double* G,
-                                   long tn, double mini, unsigned long long* masks, int* cnt, const int* indptr,
+                                   int jm, int row, long tr, long nt, int e, const PF& pf, int qr, bool has_cd,
+                                   const double* G, long tn, double mini, unsigned long long* masks, int* cnt,
                                    int* indices, double* data, int* changed, int* bad) {
   constexpr int NA = 7 * NZ;
   const int lane = threadIdx.x % 64;
@@ -171,14 +225,7 @@ __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T,
   double v[CSR_MASK_WORDS];
   unsigned long long mk[CSR_MASK_WORDS];
   bool oob = false;
-  // the stored masks and the row start, loaded ahead of the values
-  unsigned long long old[CSR_MASK_WORDS];
-  int pos0 = 0;
-  if constexpr (MODE == JR_REFRESH) {
-#pragma unroll
-    for (int w = 0; w < CSR_MASK_WORDS; ++w) old[w] = masks[w * nt + tr];
-  }
-  if constexpr (MODE != JR_COUNT) pos0 = indptr[row];
+  const double cd = pf.cd(qr);
   // the lane's slots (word w, zone n): window indices first, then every
   // ghost-part load issued at once (from entry 0 where a slot has none, the
   // branch on tn being row-uniform), so that they are in flight together
@@ -240,10 +287,10 @@ __device__ inline void jr_row_edge(const double* J, const JcTables<NZ, true>& T,
   bool same = true;
   if constexpr (MODE == JR_REFRESH) {
 #pragma unroll
-    for (int w = 0; w < CSR_MASK_WORDS; ++w) same = (old[w] == mk[w]) & same;
+    for (int w = 0; w < CSR_MASK_WORDS; ++w) same = (pf.mask(qr, e, w) == mk[w]) & same;
   }
   if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
-    int pos = pos0;
+    int pos = pf.start(qr, e);
 #pragma unroll
     for (int w = 0; w < CSR_MASK_WORDS; ++w) {
       if ((mk[w] >> lane) & 1ull) {
@@ -303,22 +350,15 @@ __device__ inline PlainLane plain_lane(int j) {
   return pl;
 }
 
-template <int NZ, int MODE>
+template <int NZ, int MODE, class PF>
 __device__ inline void jr_row_plain(const PlainLane& pl, const double* J, int r, int i, int j, int jm, int row,
-                                    long tr, long nt, int e, double cd, bool has_cd, double mini,
-                                    unsigned long long* masks, int* cnt, const int* indptr, int* indices,
-                                    double* data, int* changed) {
+                                    long tr, long nt, int e, const PF& pf, int qr, bool has_cd, double mini,
+                                    unsigned long long* masks, int* cnt, int* indices, double* data,
+                                    int* changed) {
   const int lane = threadIdx.x % 64;
   double v[JR_PLAIN_WORDS];
   unsigned long long mk[JR_PLAIN_WORDS];
-  // the stored masks and the row start, loaded ahead of the values
-  unsigned long long old[JR_PLAIN_WORDS];
-  int pos0 = 0;
-  if constexpr (MODE == JR_REFRESH) {
-#pragma unroll
-    for (int w = 0; w < JR_PLAIN_WORDS; ++w) old[w] = masks[w * nt + tr];
-  }
-  if constexpr (MODE != JR_COUNT) pos0 = indptr[row];
+  const double cd = pf.cd(qr);
 #pragma unroll
   for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
     double acc = 0.0;
@@ -346,10 +386,10 @@ __device__ inline void jr_row_plain(const PlainLane& pl, const double* J, int r,
   bool same = true;
   if constexpr (MODE == JR_REFRESH) {
 #pragma unroll
-    for (int w = 0; w < JR_PLAIN_WORDS; ++w) same = (old[w] == mk[w]) & same;
+    for (int w = 0; w < JR_PLAIN_WORDS; ++w) same = (pf.mask(qr, e, w) == mk[w]) & same;
   }
   if (MODE == JR_FILL || (MODE == JR_REFRESH && same)) {
-    int pos = pos0;
+    int pos = pf.start(qr, e);
 #pragma unroll
     for (int w = 0; w < JR_PLAIN_WORDS; ++w) {
       if ((mk[w] >> lane) & 1ull) {
@@ -487,6 +527,8 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
     if (!any) return;
   }
   const PlainLane pln = plain_lane(j);
+  RowPrefetch<MODE, EDGE ? CSR_MASK_WORDS : JR_PLAIN_WORDS, RPW> pf;
+  pf.load(m, i0, j, wave, NWAVE, masks, indptr, coefdiag);
 
 #pragma unroll 1
   for (int e = 0; e < 5; ++e) {
@@ -518,16 +560,15 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
       if (i > im) break;
       const int row = e + (j - 1) * 5 + (i - 1) * jm * 5;
       const long tr = csr_tid(m, i, j, e);
-      const double cd = coefdiag ? coefdiag[(i - 1) + (long)im * (j - 1)] : 0.0;
       const long tn = near_index(m, i, j);
       if constexpr (!EDGE) {
         if (plain[r])
-          jr_row_plain<NZ, MODE>(pln, J, r, i, j, jm, row, tr, nt, e, cd, coefdiag != nullptr, mini, masks, cnt,
-                                 indptr, indices, data, changed);
+          jr_row_plain<NZ, MODE>(pln, J, r, i, j, jm, row, tr, nt, e, pf, qr, coefdiag != nullptr, mini, masks,
+                                 cnt, indices, data, changed);
       } else {
         if (!plain[r])
-          jr_row_edge<NZ, MODE>(J, T, z, r, i, jm, row, tr, nt, e, cd, coefdiag != nullptr, G, tn, mini, masks,
-                                cnt, indptr, indices, data, changed, bad);
+          jr_row_edge<NZ, MODE>(J, T, z, r, i, jm, row, tr, nt, e, pf, qr, coefdiag != nullptr, G, tn, mini,
+                                masks, cnt, indices, data, changed, bad);
       }
     }
     __syncthreads();
