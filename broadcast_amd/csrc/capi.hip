@@ -1003,8 +1003,8 @@ int jacobian_impl(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p,
   BCA_LAUNCHED();
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<false><<<dim3(grid1(near_count(m), 256), nslot * 5), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
-                                                                              c->ghostnz, jac);
+    k_assemble_ghosts<<<dim3(grid1(near_count(m), 256), nslot * 5), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
+                                                                       c->ghostnz, jac);
   }
   BCA_LAUNCHED();
   return 0;
@@ -1527,9 +1527,8 @@ int jac_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const
   }
   {
     Timed t(c, "k_assemble_ghosts");
-    k_assemble_ghosts<true><<<dim3(grid1(near_count(m), 256), nslot * 5), 256, 0, s>>>(m, z, c->blki, c->blkj,
-                                                                                     c->ghostT, c->ghostnz,
-                                                                                     c->ghostG);
+    k_ghost_part<<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
+                                                                      c->ghostnz, c->ghostG);
   }
   BCA_LAUNCHED();
   const int nb = (int)((rows + SCAN_B - 1) / SCAN_B);

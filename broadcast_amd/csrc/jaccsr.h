@@ -8,7 +8,7 @@
 // Every value is the COO's bit for bit: the sum k_assemble_colored writes to
 // a slot (0 - left i-face + right i-face - lower j-face + upper j-face block
 // entries), + coefdiag on the diagonal, + the ghost part (compact array of
-// k_assemble_ghosts<true>) on rows near the edge, in that order.
+// k_ghost_part) on rows near the edge, in that order.
 //
 // k_jac_csr_rows works on tiles of JC_TI consecutive i of one j: the tile's
 // Jacobian rows of one variable e are summed in LDS from the face blocks

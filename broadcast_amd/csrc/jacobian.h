@@ -314,11 +314,8 @@ __device__ inline void ghost_face(const Mesh& m, const double* blk, long nf, lon
 }
 
 // Grid (x: near rows, y: (zone * 49 + color) * 5 + e), one thread per near
-// row cell, slot color and variable row e.  COMPACT = false adds the ghost
-// part to the COO slots; COMPACT = true stores it as
-// G[(t * nslot + zone * 49 + color) * 25 + e * 5 + m] for the fused CSR path
-// (zeros for a slot outside the zone's window).
-template <bool COMPACT>
+// row cell, slot color and variable row e: adds the ghost part to the COO
+// slots.
 __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const double* __restrict__ bi,
                                                          const double* __restrict__ bj,
                                                          const double* __restrict__ T,
@@ -327,20 +324,13 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= near_count(m)) return;
   const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
-  const int slot = blockIdx.y / 5, e = blockIdx.y % 5, nslot = gridDim.y / 5;
+  const int slot = blockIdx.y / 5, e = blockIdx.y % 5;
   const int color = slot % (step * step), zn = slot / (step * step);
   const int l = color / step, k = color % step;
   int i, j;
   near_cell(m, t, i, j);
   int vali, valj;
-  double* gout = jac + (t * nslot + slot) * 25 + e * 5;
-  if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) {
-    if (COMPACT) {
-#pragma unroll
-      for (int mm = 0; mm < 5; ++mm) gout[mm] = 0.0;
-    }
-    return;
-  }
+  if (!zone_column(m, z, zn, i, j, l, k, vali, valj)) return;
   const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
   const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
   const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
@@ -352,16 +342,83 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   ghost_face<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, e, acc);
   ghost_face<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, e, acc);
   ghost_face<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, e, acc);
-  if (COMPACT) {
-#pragma unroll
-    for (int mm = 0; mm < 5; ++mm) gout[mm] = acc[mm];
-    return;
-  }
   const long plane = (long)im * jm;
   const long base = (long)(i - 1) + (long)(j - 1) * im + (long)k * plane * 5 + (long)l * plane * 5 * step +
                     (long)zn * plane * 25 * step * step + (long)e * plane;
 #pragma unroll
   for (int mm = 0; mm < 5; ++mm) jac[base + (long)mm * plane * 5 * step * step] += acc[mm];
+}
+
+// ghost_face for the five variable rows at once: the face's ghost offsets
+// are found once, each contribution reads its 25 tangent values once; per
+// row e the terms and their order are those of ghost_face.
+template <bool JFAM>
+__device__ inline void ghost_face_rows(const Mesh& m, const double* blk, long nf, long f, int fi, int fj,
+                                       double sg, const double* Tc, const unsigned char* nzc, double acc[25]) {
+  for (int o = 0; o < NOFF; ++o) {
+    if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
+    const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
+    const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
+    if (interior(m, ci, cj)) continue;
+    if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
+    const long g = ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
+    if (!nzc[g]) continue;
+    const double* tg = Tc + g * 25;
+    double tt[25];
+#pragma unroll
+    for (int x = 0; x < 25; ++x) tt[x] = tg[x];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) {
+      const double* b = blk + ((long)o * 25 + e * 5) * nf + f;
+      double bb[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) bb[q] = b[q * nf];
+#pragma unroll
+      for (int mm = 0; mm < 5; ++mm) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s += bb[q] * tt[q * 5 + mm];
+        acc[e * 5 + mm] += sg * s;
+      }
+    }
+  }
+}
+
+// The ghost part in the compact array of the fused CSR path:
+// G[(t * nslot + zone * 49 + color) * 25 + e * 5 + m], zeros for a slot
+// outside the zone's window.  Grid (x: near rows, y: slot); one thread per
+// near row cell and slot forms its five variable rows (the face and ghost
+// search done once for them) and stores its 25 values as one run.
+__global__ void __launch_bounds__(256) k_ghost_part(Mesh m, Zones z, const double* __restrict__ bi,
+                                                    const double* __restrict__ bj, const double* __restrict__ T,
+                                                    const unsigned char* __restrict__ nz, double* __restrict__ G) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= near_count(m)) return;
+  const int im = m.im, jm = m.jm, gh = m.gh, step = 2 * gh + 1;
+  const int slot = blockIdx.y, nslot = gridDim.y;
+  const int color = slot % (step * step), zn = slot / (step * step);
+  const int l = color / step, k = color % step;
+  int i, j;
+  near_cell(m, t, i, j);
+  int vali, valj;
+  double* gout = G + (t * nslot + slot) * 25;
+  double acc[25];
+#pragma unroll
+  for (int x = 0; x < 25; ++x) acc[x] = 0.0;
+  if (zone_column(m, z, zn, i, j, l, k, vali, valj)) {
+    const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
+    const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
+    const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
+    const long ng = ghost_count(m);
+    const double* Tc = T + (long)slot * ng * 25;
+    const unsigned char* nzc = nz + (long)slot * ng;
+    ghost_face_rows<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, nzc, acc);
+    ghost_face_rows<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, acc);
+    ghost_face_rows<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, acc);
+    ghost_face_rows<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, acc);
+  }
+#pragma unroll
+  for (int x = 0; x < 25; ++x) gout[x] = acc[x];
 }
 
 }  // namespace bca
