@@ -349,46 +349,26 @@ __global__ void __launch_bounds__(256) k_assemble_ghosts(Mesh m, Zones z, const 
   for (int mm = 0; mm < 5; ++mm) jac[base + (long)mm * plane * 5 * step * step] += acc[mm];
 }
 
-// ghost_face for the five variable rows at once: the face's ghost offsets
-// are found once, each contribution reads its 25 tangent values once; per
-// row e the terms and their order are those of ghost_face.
-template <bool JFAM>
-__device__ inline void ghost_face_rows(const Mesh& m, const double* blk, long nf, long f, int fi, int fj,
-                                       double sg, const double* Tc, const unsigned char* nzc, double acc[25]) {
-  for (int o = 0; o < NOFF; ++o) {
-    if (JFAM ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) continue;
-    const int ci = fi + (JFAM ? joff_di(o) : ioff_di(o));
-    const int cj = fj + (JFAM ? joff_dj(o) : ioff_dj(o));
-    if (interior(m, ci, cj)) continue;
-    if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) continue;
-    const long g = ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
-    if (!nzc[g]) continue;
-    const double* tg = Tc + g * 25;
-    double tt[25];
-#pragma unroll
-    for (int x = 0; x < 25; ++x) tt[x] = tg[x];
-#pragma unroll
-    for (int e = 0; e < 5; ++e) {
-      const double* b = blk + ((long)o * 25 + e * 5) * nf + f;
-      double bb[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) bb[q] = b[q * nf];
-#pragma unroll
-      for (int mm = 0; mm < 5; ++mm) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) s += bb[q] * tt[q * 5 + mm];
-        acc[e * 5 + mm] += sg * s;
-      }
-    }
-  }
+// The ghost cell that face f of the row cell (i, j) -- 0, 1: left and right
+// i-face, 2, 3: lower and upper j-face -- reads through its stencil offset
+// o, or -1 when that cell is not a ghost (or the offset is not live).
+__device__ inline long ghost_of(const Mesh& m, int f, int o, int i, int j) {
+  const bool jf = f >= 2;
+  const int fi = i + (f == 1 ? 1 : 0), fj = j + (f == 3 ? 1 : 0);
+  if (jf ? !joff_live(m, o, fj) : !ioff_live(m, o, fi)) return -1;
+  const int ci = fi + (jf ? joff_di(o) : ioff_di(o));
+  const int cj = fj + (jf ? joff_dj(o) : ioff_dj(o));
+  if (interior(m, ci, cj)) return -1;
+  if (ci < 1 - m.gh || ci > m.im + m.gh || cj < 1 - m.gh || cj > m.jm + m.gh) return -1;
+  return ghost_index(m, ci + m.gh - 1, cj + m.gh - 1);
 }
 
 // The ghost part in the compact array of the fused CSR path:
 // G[(t * nslot + zone * 49 + color) * 25 + e * 5 + m], zeros for a slot
 // outside the zone's window.  Grid (x: near rows, y: slot); one thread per
 // near row cell and slot forms its five variable rows (the face and ghost
-// search done once for them) and stores its 25 values as one run.
+// search done once for them, its nonzero-tangent probes all in flight
+// together) and stores its 25 values as one run.
 __global__ void __launch_bounds__(256) k_ghost_part(Mesh m, Zones z, const double* __restrict__ bi,
                                                     const double* __restrict__ bj, const double* __restrict__ T,
                                                     const unsigned char* __restrict__ nz, double* __restrict__ G) {
@@ -407,15 +387,58 @@ __global__ void __launch_bounds__(256) k_ghost_part(Mesh m, Zones z, const doubl
   for (int x = 0; x < 25; ++x) acc[x] = 0.0;
   if (zone_column(m, z, zn, i, j, l, k, vali, valj)) {
     const long nfi = (long)(im + 1) * jm, nfj = (long)im * (jm + 1);
-    const long fi0 = (i - 1) + (long)(im + 1) * (j - 1), fi1 = fi0 + 1;
-    const long fj0 = (i - 1) + (long)im * (j - 1), fj1 = fj0 + im;
+    const long fi0 = (i - 1) + (long)(im + 1) * (j - 1);
+    const long fj0 = (i - 1) + (long)im * (j - 1);
     const long ng = ghost_count(m);
     const double* Tc = T + (long)slot * ng * 25;
     const unsigned char* nzc = nz + (long)slot * ng;
-    ghost_face_rows<false>(m, bi, nfi, fi0, i, j, -1.0, Tc, nzc, acc);
-    ghost_face_rows<false>(m, bi, nfi, fi1, i + 1, j, +1.0, Tc, nzc, acc);
-    ghost_face_rows<true>(m, bj, nfj, fj0, i, j, -1.0, Tc, nzc, acc);
-    ghost_face_rows<true>(m, bj, nfj, fj1, i, j + 1, +1.0, Tc, nzc, acc);
+    // (face, offset) pairs whose ghost has a nonzero tangent, bit f NOFF + o:
+    // every probe issued at once (from entry 0 where there is no ghost)
+    unsigned long long mk[2] = {0ull, 0ull};
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+#pragma unroll
+      for (int o = 0; o < NOFF; ++o) {
+        const long g = ghost_of(m, f, o, i, j);
+        const bool live = (g >= 0) & (nzc[g >= 0 ? g : 0] != 0);
+        const int b = f * NOFF + o;
+        mk[b / 64] |= (unsigned long long)live << (b % 64);
+      }
+    }
+    // the terms in the order of ghost_face: faces left, right, lower,
+    // upper, offsets ascending within each
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      unsigned long long bits = mk[h];
+      while (bits) {
+        const int b = h * 64 + __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        const int f = b / NOFF, o = b % NOFF;
+        const long g = ghost_of(m, f, o, i, j);
+        const double sg = (f == 0 || f == 2) ? -1.0 : 1.0;
+        const double* blk = f < 2 ? bi : bj;
+        const long nf = f < 2 ? nfi : nfj;
+        const long fc = f == 0 ? fi0 : f == 1 ? fi0 + 1 : f == 2 ? fj0 : fj0 + im;
+        const double* tg = Tc + g * 25;
+        double tt[25];
+#pragma unroll
+        for (int x = 0; x < 25; ++x) tt[x] = tg[x];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+          const double* bp = blk + ((long)o * 25 + e * 5) * nf + fc;
+          double bb[5];
+#pragma unroll
+          for (int q = 0; q < 5; ++q) bb[q] = bp[q * nf];
+#pragma unroll
+          for (int mm = 0; mm < 5; ++mm) {
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) sum += bb[q] * tt[q * 5 + mm];
+            acc[e * 5 + mm] += sg * sum;
+          }
+        }
+      }
+    }
   }
 #pragma unroll
   for (int x = 0; x < 25; ++x) gout[x] = acc[x];
