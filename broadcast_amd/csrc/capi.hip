@@ -1552,7 +1552,7 @@ int jac_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const
     k_jac_csr_rows<NZ, MODE, false><<<gr, JR_THREADS, 0, s>>>(m, z, c->blki, c->blkj, coefdiag, c->ghostG,    \
                                                               mini, c->jr_mask, c->jr_cnt, indptr, indices,   \
                                                               data, changed, bad);                            \
-    k_jac_csr_rows<NZ, MODE, true><<<(unsigned)jr_edge_tiles(gr.x, m.jm), JR_THREADS, 0, s>>>(                 \
+    k_jac_csr_rows<NZ, MODE, true><<<(unsigned)jr_edge_tiles(gr.x, m.jm), JR_EDGE_THREADS, 0, s>>>(            \
         m, z, c->blki, c->blkj, coefdiag, c->ghostG, mini, c->jr_mask, c->jr_cnt, indptr, indices, data, changed, \
         bad);                                                                                                  \
   } while (0)

@@ -139,7 +139,8 @@ __device__ inline void jc_tables(const Mesh& m, const Zones& z, int i0, int j, J
 // eight in column order otherwise) depends only on the mesh and the zones,
 // so all three modes agree on it.
 enum { JR_COUNT = 0, JR_FILL = 1, JR_REFRESH = 2 };
-constexpr int JR_THREADS = 512;
+constexpr int JR_THREADS = 512;       // plain launch
+constexpr int JR_EDGE_THREADS = 256;  // edge launch
 constexpr int JR_PITCH = JC_TI + 1;
 // The cells (di, dj) relative to a row cell that its four faces' stencils
 // reach: the 29-cell stencil of the interior plus (0, 4), which the row j = 1
@@ -153,7 +154,6 @@ __host__ __device__ constexpr int jr_cell_dj(int c) {
   return c < 1 ? 0 : c < 6 ? c - 3 : c < 11 ? c - 8 : c < 19 ? c - 14 : c < 24 ? c - 21 : c < 29 ? c - 26 : 0;
 }
 constexpr int JR_NELEM = JC_TI * JR_NCELL * 5;                   // (il, m, cell) value items of one e
-constexpr int JR_NIT = (JR_NELEM + JR_THREADS - 1) / JR_THREADS;
 
 // The row phase's loads of a wave's rows -- the stored mask words and the
 // row start of each variable row, coefdiag of each row cell -- issued once
@@ -444,7 +444,7 @@ __device__ inline void jr_edge_tile(long t, int nxt, int jm, int& x, int& j) {
 }
 
 template <int NZ, int MODE, bool EDGE>
-__global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, const double* __restrict__ bi,
+__global__ void __launch_bounds__(EDGE ? JR_EDGE_THREADS : JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, const double* __restrict__ bi,
                                                              const double* __restrict__ bj,
                                                              const double* __restrict__ coefdiag,
                                                              const double* __restrict__ G, double mini,
@@ -453,7 +453,9 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
                                                              int* __restrict__ indices, double* __restrict__ data,
                                                              int* __restrict__ changed, int* __restrict__ bad) {
   constexpr int NA = 7 * NZ;
-  constexpr int NWAVE = JR_THREADS / 64;
+  constexpr int TH = EDGE ? JR_EDGE_THREADS : JR_THREADS;
+  constexpr int JR_NIT = (JR_NELEM + TH - 1) / TH;
+  constexpr int NWAVE = TH / 64;
   constexpr int RPW = JC_TI / NWAVE;                             // rows per wave
   constexpr int STEP = 7;                                        // 2 gh + 1, gh = 3
   __shared__ double J[245 * JR_PITCH];                           // (k, D, m) x il
@@ -471,7 +473,7 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
   if (tid < 7) T.inv_k[tid] = -1;
   jc_tables<NZ, true>(m, z, i0, j, T);
   // window entries no face reaches stay zero for every e
-  for (int x = tid; x < 245 * JR_PITCH; x += JR_THREADS) J[x] = 0.0;
+  for (int x = tid; x < 245 * JR_PITCH; x += TH) J[x] = 0.0;
   __syncthreads();
   const int wave = tid / 64, lane = tid % 64;
 
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(JR_THREADS) k_jac_csr_rows(Mesh m, Zones z, co
   int vdst[JR_NIT];
 #pragma unroll
   for (int it = 0; it < JR_NIT; ++it) {
-    const int item = tid + it * JR_THREADS;
+    const int item = tid + it * TH;
     const int il = item % JC_TI, rest = item / JC_TI, mm = rest % 5, cell = rest / 5;
     const int i = i0 + il;
     vdst[it] = -1;
