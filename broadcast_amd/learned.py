@@ -185,6 +185,139 @@ class GuessNet:
         return y.double() * self.scale.double()
 
 
+class _MatMul:
+    """Autograd product on the matrix cores: C = A B (fp32 out) with bf16
+    operands, the backward products dA = dC B^T and dB = A^T dC through the
+    same kernel (split-K for the long reductions)."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, a, b):
+        if cls._fn is None:
+            t = dev.torch()
+
+            class Fn(t.autograd.Function):
+                @staticmethod
+                def forward(ctx, a, b):
+                    ab, bb = a.to(t.bfloat16), b.to(t.bfloat16)
+                    ctx.save_for_backward(ab, bb)
+                    return mm(ab, bb, ksplit=_ksplit(ab.shape[1]))
+
+                @staticmethod
+                def backward(ctx, g):
+                    ab, bb = ctx.saved_tensors
+                    gb = g.to(t.bfloat16)
+                    da = mm(gb, bb.t(), ksplit=_ksplit(gb.shape[1])) if ctx.needs_input_grad[0] else None
+                    db = mm(ab.t(), gb, ksplit=_ksplit(ab.shape[0])) if ctx.needs_input_grad[1] else None
+                    return da, db
+
+            cls._fn = Fn
+        return cls._fn.apply(a, b)
+
+
+@dataclass
+class ConvGuessNet:
+    """The same trunk as GuessNet with a convolutional decoder instead of the
+    linear POD-initialised one:
+        c (K) -> coarse grid (C0 x jm/16 x im/16) by a dense layer
+        4 x [nearest 2x upsampling, 3 x 3 convolution, silu]
+        3 x 3 convolution to the 5 variables, + the mean field.
+    Padding is periodic along i (the O-mesh's cut) and replicated along j.
+    Each convolution is im2col (torch unfold) and one product on the matrix
+    cores; backward by autograd through _MatMul."""
+    n: int
+    im: int
+    jm: int
+    width: int = 64
+    modes: int = 16
+    chans: tuple = (32, 32, 16, 16, 16)
+    params: dict = field(default_factory=dict)
+    scale: object = None
+    mean: object = None
+
+    @staticmethod
+    def create(fields, im, jm, width=64, modes=16, chans=(32, 32, 16, 16, 16), seed=0):
+        t = dev.torch()
+        if im % 16 or jm % 16:
+            raise ValueError("the convolutional decoder needs im and jm divisible by 16")
+        N, n = fields.shape
+        net = ConvGuessNet(n=n, im=im, jm=jm, width=width, modes=modes, chans=tuple(chans))
+        f = fields.view(N, 5, -1)
+        s = f.abs().amax(dim=(0, 2)).clamp_min(1e-12)
+        net.scale = s.repeat_interleave(f.shape[2]).to(t.float32)
+        net.mean = (fields / net.scale.double()).mean(dim=0).to(t.float32)
+        g = t.Generator(device="cpu").manual_seed(seed)
+        rnd = lambda *shape, fan: (t.randn(*shape, generator=g) * math.sqrt(1.0 / fan)).to(fields.device)
+        c0 = chans[0]
+        cells0 = (jm // 16) * (im // 16)
+        p = {
+            "W1": rnd(width, 2, fan=2), "b1": t.zeros(width, device=fields.device),
+            "W2": rnd(width, width, fan=width), "b2": t.zeros(width, device=fields.device),
+            "W3": rnd(modes, width, fan=width), "b3": t.zeros(modes, device=fields.device),
+            "Wc": rnd(modes, c0 * cells0, fan=modes), "bc": t.zeros(c0 * cells0, device=fields.device),
+        }
+        for k in range(len(chans) - 1):
+            p[f"K{k}"] = rnd(chans[k] * 9, chans[k + 1], fan=chans[k] * 9)
+            p[f"k{k}"] = t.zeros(chans[k + 1], device=fields.device)
+        p["Kout"] = rnd(chans[-1] * 9, 5, fan=chans[-1] * 9) * 0.1
+        p["kout"] = t.zeros(5, device=fields.device)
+        for v in p.values():
+            v.requires_grad_(True)
+        net.params = p
+        return net
+
+    def parameter_count(self):
+        return sum(int(v.numel()) for v in self.params.values())
+
+    @staticmethod
+    def _conv(x, w, b):
+        """3 x 3 convolution of x (B, C, H, W): periodic in W, replicated in
+        H; w (9 C, Cout)."""
+        t = dev.torch()
+        F = t.nn.functional
+        B, Cc, H, W = x.shape
+        xp = F.pad(F.pad(x, (1, 1, 0, 0), mode="circular"), (0, 0, 1, 1), mode="replicate")
+        cols = F.unfold(xp, 3)                                     # (B, 9 C, H W)
+        cols = cols.transpose(1, 2).reshape(B * H * W, Cc * 9)
+        y = _MatMul.apply(cols, w) + b                             # (B H W, Cout)
+        return y.view(B, H, W, -1).permute(0, 3, 1, 2)
+
+    def forward_t(self, x):
+        """x (B, 2) fp32 -> normalised field (B, n), differentiable."""
+        t = dev.torch()
+        F = t.nn.functional
+        p = self.params
+        silu = F.silu
+        h = silu(_MatMul.apply(x, p["W1"].t()) + p["b1"])
+        h = silu(_MatMul.apply(h, p["W2"].t()) + p["b2"])
+        c = _MatMul.apply(h, p["W3"].t()) + p["b3"]
+        g = _MatMul.apply(c, p["Wc"]) + p["bc"]
+        B = x.shape[0]
+        z = g.view(B, self.chans[0], self.jm // 16, self.im // 16)
+        for k in range(len(self.chans) - 1):
+            z = F.interpolate(z, scale_factor=2, mode="nearest")
+            z = silu(self._conv(z, p[f"K{k}"], p[f"k{k}"]))
+        z = self._conv(z, p["Kout"], p["kout"])                    # (B, 5, jm, im)
+        return z.reshape(B, self.n) + self.mean
+
+    def loss_and_grads(self, x, target):
+        for v in self.params.values():
+            v.grad = None
+        y = self.forward_t(x)
+        diff = y - target
+        loss = (diff * diff).mean()
+        loss.backward()
+        return float(loss.detach()), {k: v.grad for k, v in self.params.items()}
+
+    def predict(self, params_phys):
+        t = dev.torch()
+        x = t.as_tensor(scale_params(params_phys), dtype=t.float32, device=self.mean.device)
+        with t.no_grad():
+            y = self.forward_t(x.reshape(-1, 2))
+        return y.double() * self.scale.double()
+
+
 def train(net, params_phys, fields, steps=2000, lr=3e-3, log=None):
     """Adam on the FP32 master weights against the normalised training fields
     (fields: N x n FP64 device tensor).  Returns the loss history."""
@@ -199,12 +332,13 @@ def train(net, params_phys, fields, steps=2000, lr=3e-3, log=None):
         loss, g = net.loss_and_grads(x, target)
         hist.append(loss)
         a = lr * (0.5 * (1.0 + math.cos(math.pi * it / steps)) * 0.9 + 0.1)   # cosine to lr/10
-        for k, p in net.params.items():
-            m[k].mul_(b1).add_(g[k], alpha=1 - b1)
-            v[k].mul_(b2).addcmul_(g[k], g[k], value=1 - b2)
-            mh = m[k] / (1 - b1 ** it)
-            vh = v[k] / (1 - b2 ** it)
-            p.sub_(a * mh / (vh.sqrt() + eps))
+        with t.no_grad():
+            for k, p in net.params.items():
+                m[k].mul_(b1).add_(g[k], alpha=1 - b1)
+                v[k].mul_(b2).addcmul_(g[k], g[k], value=1 - b2)
+                mh = m[k] / (1 - b1 ** it)
+                vh = v[k] / (1 - b2 ** it)
+                p.sub_(a * mh / (vh.sqrt() + eps))
         if log is not None and (it % max(1, steps // 10) == 0 or it == 1):
             log(f"train step {it}: loss {loss:.3e}")
     return hist
@@ -439,6 +573,76 @@ def experiment(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, se
     }
 
 
+def compare_decoders(im, jm, n_train=96, n_test=16, width=64, modes=16, steps=3000, seed=4, log=print,
+                     progress=None):
+    """The linear POD-initialised decoder (GuessNet) against the
+    convolutional one (ConvGuessNet), both trained on the same baseflows with
+    the same trunk, steps and schedule; for each held-out point Newton from
+    the free stream and from each guess.  Returns a JSON-able summary."""
+    t = dev.torch()
+    params = sobol_params(n_train + n_test, seed=seed)
+    train_p, test_p = params[:n_train], params[n_train:]
+    t0 = time.perf_counter()
+    fields, stats, train_p = baseflows(im, jm, train_p, log=log)
+    gen_s = time.perf_counter() - t0
+    nets, info = {}, {}
+    for name in ("linear", "conv"):
+        if name == "linear":
+            net = GuessNet.create(fields, width=width, modes=modes, seed=seed)
+        else:
+            net = ConvGuessNet.create(fields, im, jm, width=width, modes=modes, seed=seed)
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = train(net, train_p, fields, steps=steps, log=log)
+        t.cuda.synchronize()
+        nets[name] = net
+        info[name] = {"parameters": net.parameter_count(), "training_s": time.perf_counter() - t0,
+                      "loss_first_last": [hist[0], hist[-1]]}
+        log(f"{name} decoder: {net.parameter_count()} parameters, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    rows = []
+    for re, ma in test_p:
+        f = make_flow(im, jm, re, ma)
+        plain, w0 = solve_plain(f)
+        sol = interior_field(f, w0)
+        row = {"re": float(re), "mach": float(ma),
+               "plain": {"seconds": plain.seconds, "newton_steps": plain.iterations, "converged": plain.converged}}
+        for name, net in nets.items():
+            t.cuda.synchronize()
+            t1 = time.perf_counter()
+            guess = guess_field(f, net, re, ma)
+            t.cuda.synchronize()
+            infer_s = time.perf_counter() - t1
+            dist = float((guess - sol).norm() / sol.norm())
+            s, _ = solve_from_guess(f, guess)
+            row[name] = {"seconds": s.seconds + infer_s, "newton_steps": s.iterations, "converged": s.converged,
+                         "start": s.start, "guess_distance": dist}
+        rows.append(row)
+        log(f"held out Re {re:.1f} M {ma:.3f}: plain {plain.iterations} steps {plain.seconds:.2f} s; "
+            + "; ".join(f"{n} {row[n]['newton_steps']} steps {row[n]['seconds']:.2f} s "
+                        f"(distance {row[n]['guess_distance']:.2e})" for n in nets))
+        if progress is not None:
+            progress(rows)
+    out = {"mesh": [im, jm], "train": int(len(train_p)), "held_out": n_test, "sobol_seed": seed,
+           "trunk": {"width": width, "modes": modes}, "training_steps": steps, "baseflow_generation_s": gen_s,
+           "held_out_results": rows}
+    ps = np.array([r["plain"]["seconds"] for r in rows])
+    for name in nets:
+        ok = np.array([r["plain"]["converged"] and r[name]["converged"] for r in rows], dtype=bool)
+        ls = np.array([r[name]["seconds"] for r in rows])
+        out[name] = dict(info[name], **{
+            "guess_distance_median": float(np.median([r[name]["guess_distance"] for r in rows])),
+            "guess_distance_max": float(np.max([r[name]["guess_distance"] for r in rows])),
+            "newton_steps_mean": float(np.mean([r[name]["newton_steps"] for r in rows])),
+            "seconds_total": float(ls.sum()),
+            "speedup_wall_converged": float(ps[ok].sum() / ls[ok].sum()) if ok.any() else None,
+            "converged_points": int(ok.sum()),
+            "fallbacks": int(sum(r[name]["start"] != "learned" for r in rows)),
+        })
+    out["plain"] = {"newton_steps_mean": float(np.mean([r["plain"]["newton_steps"] for r in rows])),
+                    "seconds_total": float(ps.sum())}
+    return out
+
+
 def main(argv=None):
     import argparse
     import json
@@ -452,6 +656,8 @@ def main(argv=None):
     ap.add_argument("--modes", type=int, default=16)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--compare-decoders", action="store_true",
+                    help="train the linear and the convolutional decoder on the same baseflows and compare them")
     a = ap.parse_args(argv)
     def progress(rows):
         # the held-out rows so far, so that a run cut short keeps them
@@ -459,8 +665,9 @@ def main(argv=None):
             with open(a.out, "w") as fh:
                 json.dump({"partial": True, "mesh": [a.im, a.jm], "held_out_results": rows}, fh, indent=1)
 
-    res = experiment(a.im, a.jm, a.train, a.test, a.width, a.modes, a.steps, log=lambda s: print(s, flush=True),
-                     progress=progress)
+    run = compare_decoders if a.compare_decoders else experiment
+    res = run(a.im, a.jm, a.train, a.test, a.width, a.modes, a.steps, log=lambda s: print(s, flush=True),
+              progress=progress)
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

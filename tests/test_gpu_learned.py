@@ -9,6 +9,12 @@ hand-written backward, and the guess against the free-stream start.
   kernel, BF16 operands) against torch autograd on the CPU in float64 of the
   same network, normwise per parameter within 3e-2 (BF16 operands carry
   8 bits; a wrong transpose or term is O(1) off).
+* Convolutional decoder: its gradients (autograd through the MFMA product,
+  BF16 operands) against the same network on the CPU in float64 with the
+  same bf16 roundings, normwise per parameter within 1e-2 (only the FP32
+  accumulation differs; a wrong transpose or term is O(1) off), and its
+  padding periodic in i (a field shifted
+  along i through the decoder's last convolution shifts the output).
 * Guess: trained on Newton baseflows of a small O-mesh, on held-out (Re, M)
   it is at most half as far from the converged baseflow as the free stream
   (relative 2-norm of the interior field), and Newton from it reaches the
@@ -77,6 +83,58 @@ def test_network_backward_matches_cpu_autograd(torch_mod):
         gg, rr = grads[k].cpu().double(), P[k].grad
         err = float((gg - rr).norm() / max(float(rr.norm()), 1e-30))
         assert err <= 3e-2, (k, err)
+
+
+def test_conv_decoder_backward_matches_cpu_autograd(torch_mod):
+    import broadcast_amd.learned as L
+
+    t = torch_mod
+    g = t.Generator().manual_seed(11)
+    im, jm, N = 32, 16, 6
+    n = 5 * im * jm
+    fields = (t.randn(N, n, generator=g, dtype=t.float64) * 0.1 + 1.0).cuda()
+    net = L.ConvGuessNet.create(fields, im, jm, width=16, modes=4, chans=(8, 8, 4, 4, 4), seed=2)
+    x = (t.rand(N, 2, generator=g) * 2 - 1).cuda()
+    target = (fields / net.scale.double()).float()
+    loss, grads = net.loss_and_grads(x, target)
+    # the same network on the CPU in float64, each product rounding its
+    # operands and, in backward, the incoming gradient to bf16 where the
+    # matrix-core path does (then only the FP32 accumulation differs)
+    cpu = L.ConvGuessNet(n=n, im=im, jm=jm, width=16, modes=4, chans=(8, 8, 4, 4, 4))
+    cpu.params = {k: v.detach().cpu().double().requires_grad_(True) for k, v in net.params.items()}
+    cpu.mean = net.mean.cpu().double()
+    rb = lambda a: a.to(t.bfloat16).double()
+
+    class RefMM(t.autograd.Function):
+        @staticmethod
+        def forward(ctx, a, b):
+            ctx.save_for_backward(rb(a), rb(b))
+            return rb(a) @ rb(b)
+
+        @staticmethod
+        def backward(ctx, gr):
+            ab, bb = ctx.saved_tensors
+            return rb(gr) @ bb.t(), ab.t() @ rb(gr)
+
+    orig = L._MatMul.__dict__["apply"]
+    try:
+        L._MatMul.apply = staticmethod(RefMM.apply)
+        ref_loss = ((cpu.forward_t(x.cpu().double()) - target.cpu().double()) ** 2).mean()
+    finally:
+        L._MatMul.apply = orig
+    ref_loss.backward()
+    assert abs(loss - float(ref_loss)) <= 1e-3 * float(ref_loss)
+    for k, v in cpu.params.items():
+        gg, rr = grads[k].cpu().double(), v.grad
+        err = float((gg - rr).norm() / max(float(rr.norm()), 1e-30))
+        assert err <= 1e-2, (k, err)
+    # periodic padding along i: shifting the input of a convolution by s
+    # cells along i shifts its output by s cells
+    z = t.randn(2, 4, jm, im, generator=g).cuda()
+    w, b = net.params["Kout"].detach(), net.params["kout"].detach()
+    y0 = L.ConvGuessNet._conv(z, w, b)
+    y1 = L.ConvGuessNet._conv(t.roll(z, 5, dims=3), w, b)
+    assert float((t.roll(y0, 5, dims=3) - y1).abs().max()) <= 1e-5 * float(y0.abs().max())
 
 
 @pytest.mark.skip(reason="kept off the GPU: its one run on the current code ended with an illegal memory access "
