@@ -613,8 +613,9 @@ def vjp_figure(im=1024, jm=256, reps=10):
           file=sys.stderr, flush=True)
     return {"cells": cells, "vjp_ms": ms_v, "mcells_per_s": cells / (ms_v * 1e-3) / 1e6, "achieved_gbs": gbs,
             "frac": gbs / HBM_PEAK_GBS, "vjp_with_bc_adjoints_ms": ms_vb,
-            "workload": "flux adjoint (24 seeded passes per face contracted with the face adjoint, fixed-order "
-                        "gather); then the BC adjoints in reverse order"}
+            "workload": "flux adjoint (interior faces by a reverse sweep per face, the edge rows by 24 seeded "
+                        "passes per face contracted with the face adjoint, fixed-order gather); then the BC "
+                        "adjoints in reverse order"}
 
 
 def learned_figure():

@@ -30,5 +30,7 @@ void launch_jface_vjp_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_vjp_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_vjp_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_vjp_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_iface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 
 }  // namespace bca

@@ -1,9 +1,11 @@
 // One face-kernel variant per compilation (selected by BCA_VARIANT): the 24
 // seeded face instantiations of each variant are large, so the build compiles
 // the variants as separate objects in parallel.  Variants 0-4 store Jacobian
-// blocks, 5-9 the VJP contributions of the same passes.
+// blocks, 5-9 the VJP contributions of the same passes, 10 the reverse-mode
+// face adjoints of the interior faces.
 #include <hip/hip_runtime.h>
 
+#include "adjface.h"
 #include "blocks.h"
 #include "blockskern.h"
 
@@ -55,8 +57,11 @@ BCA_LAUNCH(launch_jface_vjp_r3, NOFF, k_jface_blocks, 3, MODE_VJP)
 #elif BCA_VARIANT == 9
 BCA_LAUNCH(launch_jface_vjp_r2, NOFF, k_jface_blocks, 2, MODE_VJP)
 BCA_LAUNCH(launch_jface_vjp_wall, NOFF, k_jface_blocks, 1, MODE_VJP)
+#elif BCA_VARIANT == 10
+BCA_LAUNCH(launch_iface_vjp_rev, REV_GROUP, k_face_reverse, false)
+BCA_LAUNCH(launch_jface_vjp_rev, REV_GROUP, k_face_reverse, true)
 #else
-#error "BCA_VARIANT must be 0..9"
+#error "BCA_VARIANT must be 0..10"
 #endif
 
 }  // namespace bca

@@ -1071,12 +1071,12 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 3, m.jm};
   {
     Timed t(c, "k_iface_vjp");
-    launch_iface_vjp_o4(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
+    launch_iface_vjp_rev(grid2(m.im + 1, m.jm - 2, b), b, s, ai);
   }
   BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpj, residub, 4, m.jm + 1};
   {
     Timed t(c, "k_jface_vjp");
-    launch_jface_vjp_c(grid2(m.im, m.jm - 2, b), b, s, aj);
+    launch_jface_vjp_rev(grid2(m.im, m.jm - 2, b), b, s, aj);
   }
   BCA_LAUNCHED();
   if ((rc = edge_join(c, es))) return rc;
