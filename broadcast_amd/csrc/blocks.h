@@ -16,6 +16,7 @@ struct BlockArgs {
   double* out;          // blocks (25 planes per offset) or VJP contributions (5)
   const double* resb;   // residual adjoint (VJP only)
   int j0, j1;
+  int i0 = 1, istep = 1;  // faces i = i0 + k istep (k = 0, 1, ...) of each row
 };
 
 void launch_iface_blocks_o4(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);

@@ -11,7 +11,7 @@
 
 namespace bca {
 
-#define BCA_ARGS a.m, a.ph, a.g, a.w, a.prim, a.grad, a.out, a.resb, a.j0, a.j1
+#define BCA_ARGS a.m, a.ph, a.g, a.w, a.prim, a.grad, a.out, a.resb, a.j0, a.j1, a.i0, a.istep
 
 // launch(integral_constant<int, o0>) for o0 = 0, GS, 2 GS, ... < NOFF
 template <int GS, int O0 = 0, class L>

@@ -128,7 +128,7 @@ __device__ inline void face_adjoint(const Mesh& m, const double* resb, int i, in
 // single offsets (no spills, 24 launches) 0.52 + 0.43 ms.
 constexpr int OFF_GROUP = 1;
 
-// i-faces (i, j), i in [1, im+1], rows j in [j0, j1]; O4 selects the
+// i-faces (i, j), i = i0, i0 + istep, ... <= im+1, rows j in [j0, j1]; O4 selects the
 // compact viscous flux (rows j >= 3) over the 2nd-order one (rows 1, 2).
 // The two extra offsets (3, 0), (-4, 0) only exist for the faces i = 1 and
 // i = im+1 (extrapolated ghost gradients); elsewhere nothing is stored for
@@ -138,10 +138,10 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
                                                       double* __restrict__ out, const double* __restrict__ resb,
-                                                      int j0, int j1) {
+                                                      int j0, int j1, int i0, int istep) {
   int bx, by;
   xcd_block(bx, by);
-  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int i = i0 + (bx * (int)blockDim.x + (int)threadIdx.x) * istep;
   const int j = j0 + by * blockDim.y + threadIdx.y;
   if (i > m.im + 1 || j > j1) return;
   const long nf = (long)(m.im + 1) * m.jm;
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_iface_blocks(Mesh m, Phys ph, Metric g,
   });
 }
 
-// j-faces (i, j), i in [1, im], one row variant per launch: ROW 0 = centred
+// j-faces (i, j), i = i0, i0 + istep, ... <= im, one row variant per launch: ROW 0 = centred
 // (rows j >= 4), 3, 2 = off-centred near-wall rows, 1 = the wall face.
 // Extra offsets: (0, 3) only for row 2, (0, -4) only for row jm+1; the wall
 // face depends on (0, 0) and (0, 1) only.
@@ -168,10 +168,10 @@ __global__ void __launch_bounds__(256) k_jface_blocks(Mesh m, Phys ph, Metric g,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad,
                                                       double* __restrict__ out, const double* __restrict__ resb,
-                                                      int j0, int j1) {
+                                                      int j0, int j1, int i0, int istep) {
   int bx, by;
   xcd_block(bx, by);
-  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int i = i0 + (bx * (int)blockDim.x + (int)threadIdx.x) * istep;
   const int j = j0 + by * blockDim.y + threadIdx.y;
   if (i > m.im || j > j1) return;
   const long nf = (long)m.im * (m.jm + 1);

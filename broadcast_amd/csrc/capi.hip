@@ -1055,19 +1055,11 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   k_grad_ghost<double><<<grid1(2 * m.im + m.jm, 256), 256, 0, s>>>(m, c->w0.grad);
   BCA_LAUNCHED();
   // edge rows on the side streams while the interior runs (disjoint face
-  // rows of vjpi / vjpj; residub is only read until the join)
+  // rows of vjpi / vjpj; residub is only read until the join); the interior
+  // kernels are queued first, so that a side stream sharing a hardware
+  // queue with c->stream does not hold them back
   hipStream_t es[3];
   if ((rc = edge_fork(c, es))) return rc;
-  BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 1, 2};
-  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
-  ae.out = c->vjpj;
-  ae.j0 = ae.j1 = 3;
-  launch_jface_vjp_r3(grid2(m.im, 1, b), b, es[1], ae);
-  ae.j0 = ae.j1 = 2;
-  launch_jface_vjp_r2(grid2(m.im, 1, b), b, es[2], ae);
-  ae.j0 = ae.j1 = 1;
-  launch_jface_vjp_wall(grid2(m.im, 1, b), b, es[0], ae);
-  BCA_LAUNCHED();
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 3, m.jm};
   {
     Timed t(c, "k_iface_vjp");
@@ -1078,6 +1070,16 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
     Timed t(c, "k_jface_vjp");
     launch_jface_vjp_rev(grid2(m.im, m.jm - 2, b), b, s, aj);
   }
+  BCA_LAUNCHED();
+  BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 1, 2};
+  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
+  ae.out = c->vjpj;
+  ae.j0 = ae.j1 = 3;
+  launch_jface_vjp_r3(grid2(m.im, 1, b), b, es[1], ae);
+  ae.j0 = ae.j1 = 2;
+  launch_jface_vjp_r2(grid2(m.im, 1, b), b, es[2], ae);
+  ae.j0 = ae.j1 = 1;
+  launch_jface_vjp_wall(grid2(m.im, 1, b), b, es[0], ae);
   BCA_LAUNCHED();
   if ((rc = edge_join(c, es))) return rc;
   {
