@@ -27,13 +27,13 @@ def _run(cmd, verbose):
 
 
 def build(verbose=False, jobs=None):
-    """Compile the C-ABI object and the eleven face-kernel variants in parallel,
+    """Compile the C-ABI object and the seven face-kernel variants in parallel,
     then link libbroadcast_amd.so."""
     csrc = os.path.join(HERE, "csrc")
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     units = [("capi", os.path.join(csrc, "capi.hip"), [])]
-    units += [(f"blocks{v}", os.path.join(csrc, "blocks.hip"), [f"-DBCA_VARIANT={v}"]) for v in range(11)]
+    units += [(f"blocks{v}", os.path.join(csrc, "blocks.hip"), [f"-DBCA_VARIANT={v}"]) for v in range(7)]
     procs = []
     for name, src, defs in units:
         obj = os.path.join(objdir, name + ".o")

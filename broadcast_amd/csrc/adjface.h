@@ -1,6 +1,5 @@
-// Reverse-mode face adjoints of the interior faces (the VJP of the DNC5
-// residual): for an i-face with the compact viscous flux (rows j >= 3) or a
-// centred j-face (rows j >= 4), hnb(face)^T d hn(face) / d w(cell) for the 24
+// Reverse-mode face adjoints (the VJP of the DNC5 residual): for every i-face
+// and every j-face above the wall row, hnb(face)^T d hn(face) / d w(cell) for the 24
 // stencil cells, the same 5 values per offset that the seeded passes of
 // k_iface_blocks / k_jface_blocks (MODE_VJP) store, in the same planes, so
 // that k_vjp_gather reads either.  The reference's adjoint is TAPENADE's
@@ -46,12 +45,14 @@ struct IndAcc : AccBase {
   template <int DI, int DJ> __device__ double W(int) const { return (DI == SI && DJ == SJ) ? 1.0 : 0.0; }
 };
 
-// weight of the cell at offset (SI, SJ) in the 4-point face value mid(k) of
-// the compact viscous flux (0.0625 (-1, 9, 9, -1) across the face)
-template <bool J>
+// weight of the cell at offset (SI, SJ) in the face value of the viscous
+// flux: 0.0625 (-1, 9, 9, -1) across the face for the compact flux (O4),
+// 0.5 (1, 1) for the 2nd-order one
+template <bool J, bool O4>
 __host__ __device__ constexpr double mid_coef(int SI, int SJ) {
   const int along = J ? SJ : SI, across = J ? SI : SJ;
   if (across != 0) return 0.0;
+  if (!O4) return (along == 0 || along == -1) ? 0.5 : 0.0;
   return along == -2 ? -0.0625 : along == -1 ? 0.5625 : along == 0 ? 0.5625 : along == 1 ? -0.0625 : 0.0;
 }
 
@@ -63,19 +64,22 @@ __device__ inline void add_d(double wb[5], const Dual<5>& x, double c) {
 }
 __device__ inline void add_d(double[5], double, double) {}
 
-// primal of the compact viscous flux: face values and gradients
+// primal of the viscous flux (compact, O4, or 2nd-order): face values and
+// gradients
 struct ViscPrimal {
   Contour c;
   double mmu, lam, uu, vv, ww;
   double ux, vx, wx, tx, uy, vy, wy, ty;
 };
-template <bool J, class A>
+template <bool J, bool O4, class A>
 __device__ inline ViscPrimal visc_primal(const A& a) {
   ViscPrimal v;
   v.c = J ? contour_j(a) : contour_i(a);
   auto q = [&](int k) {
-    if constexpr (J) return jvals_o4(a, k);
-    else return ivals_o4(a, k);
+    if constexpr (J && O4) return jvals_o4(a, k);
+    else if constexpr (J) return jvals_o2(a, k);
+    else if constexpr (O4) return ivals_o4(a, k);
+    else return ivals_o2(a, k);
   };
   const auto qu = q(P_U), qv = q(P_V), qw = q(P_W), qt = q(P_T);
   v.ux = cgx(v.c, qu.N, qu.S, qu.O, qu.E), v.uy = cgy(v.c, qu.N, qu.S, qu.O, qu.E);
@@ -83,7 +87,9 @@ __device__ inline ViscPrimal visc_primal(const A& a) {
   v.wx = cgx(v.c, qw.N, qw.S, qw.O, qw.E), v.wy = cgy(v.c, qw.N, qw.S, qw.O, qw.E);
   v.tx = cgx(v.c, qt.N, qt.S, qt.O, qt.E), v.ty = cgy(v.c, qt.N, qt.S, qt.O, qt.E);
   auto mid = [&](int k) {
-    if constexpr (J)
+    if constexpr (!O4 && J) return 0.5 * (a.template P<0, 0>(k) + a.template P<0, -1>(k));
+    else if constexpr (!O4) return 0.5 * (a.template P<0, 0>(k) + a.template P<-1, 0>(k));
+    else if constexpr (J)
       return 0.0625 * (-a.template P<0, -2>(k) + 9.0 * a.template P<0, -1>(k) + 9.0 * a.template P<0, 0>(k) -
                        a.template P<0, 1>(k));
     else
@@ -323,16 +329,25 @@ __device__ inline void prim_reverse(const Phys& ph, double cvm1, double betas, c
 }
 
 // weight of the cell `along` cells from the face (0 and -1 adjacent) in the
-// centred 6-point Euler flux (euler_o6_i.F)
+// Euler flux: ROW 0 the centred 6-point flux (euler_o6_i.F), ROW 2, 3 the
+// off-centred j-flux of those rows over cell rows 1..5 (jeuler)
+template <int ROW>
 __host__ __device__ constexpr double euler_coef(int along) {
-  return (along == 0 || along == -1) ? Coef::c1 : (along == 1 || along == -2) ? Coef::c2
-       : (along == 2 || along == -3) ? Coef::c3 : 0.0;
+  if constexpr (ROW == 0) {
+    return (along == 0 || along == -1) ? Coef::c1 : (along == 1 || along == -2) ? Coef::c2
+         : (along == 2 || along == -3) ? Coef::c3 : 0.0;
+  } else {
+    const int r = along - (1 - ROW);
+    const double k2[5] = {12.0 / 60.0, 77.0 / 60.0, -43.0 / 60.0, 17.0 / 60.0, -3.0 / 60.0};
+    const double k3[5] = {-3.0 / 60.0, 27.0 / 60.0, 47.0 / 60.0, -13.0 / 60.0, 2.0 / 60.0};
+    return (r < 0 || r > 4) ? 0.0 : ROW == 2 ? k2[r] : k3[r];
+  }
 }
 
 // The face adjoint of one face for the offsets [O0, O1): J = false an
 // i-face (compact viscous flux), true a centred j-face.  hb: the face's
 // adjoint hnb; out: out[(o * 5 + q) * nf + f].
-template <bool J, int O0, int O1>
+template <bool J, bool O4, int ROW, int O0, int O1>
 __device__ inline void face_reverse(const Mesh& m, const Phys& ph, const Metric& g, const double* w,
                                     const double* prim, const double* grad, int i, int j, const double hb[5],
                                     double* out, long nf, long f) {
@@ -356,7 +371,7 @@ __device__ inline void face_reverse(const Mesh& m, const Phys& ph, const Metric&
     e4b -= hb[e] * ds.rspec * pr;
   }
   // viscous: hn_e -= (vis_f<e> nxl + vis_g<e> nyl) sn, e = 1..4
-  const ViscPrimal vp = visc_primal<J>(a);
+  const ViscPrimal vp = visc_primal<J, O4>(a);
   double fb[5], gb[5];
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
@@ -412,8 +427,8 @@ __device__ inline void face_reverse(const Mesh& m, const Phys& ph, const Metric&
     for (int k = 0; k < NPRIM; ++k) pb[k] = 0.0;
     constexpr int along = J ? SJ : SI, across = J ? SI : SJ;
     // centred Euler flux: c (efx_e nx + efy_e ny) of the cell, weighted by hnb_e
-    if constexpr (across == 0 && along >= -3 && along <= 2) {
-      constexpr double c = euler_coef(along);
+    if constexpr (across == 0 && euler_coef<ROW>(along) != 0.0) {
+      constexpr double c = euler_coef<ROW>(along);
       double ae[5];
 #pragma unroll
       for (int e = 0; e < 5; ++e) ae[e] = c * hb[e];
@@ -463,10 +478,12 @@ __device__ inline void face_reverse(const Mesh& m, const Phys& ph, const Metric&
     // the viscous flux: the cell's weight in each linear stencil
     {
       const auto qc = [&] {
-        if constexpr (J) return jvals_o4(ind, 0);
-        else return ivals_o4(ind, 0);
+        if constexpr (J && O4) return jvals_o4(ind, 0);
+        else if constexpr (J) return jvals_o2(ind, 0);
+        else if constexpr (O4) return ivals_o4(ind, 0);
+        else return ivals_o2(ind, 0);
       }();
-      constexpr double cm = mid_coef<J>(SI, SJ);
+      constexpr double cm = mid_coef<J, O4>(SI, SJ);
 #pragma unroll
       for (int k = 0; k < 4; ++k) pb[k] += qc.N * va.N[k] + qc.S * va.S[k] + qc.O * va.O[k] + qc.E * va.E[k];
       pb[P_U] += cm * va.mid[0];
@@ -488,10 +505,12 @@ __device__ inline void face_reverse(const Mesh& m, const Phys& ph, const Metric&
 // recomputed per launch)
 constexpr int REV_GROUP = 24;
 
-// One thread per face: i-faces (i, j), j in [j0, j1] (J = false, rows
-// j >= 3), or centred j-faces (J = true, rows j >= 4), i = i0, i0 + istep,
-// ...; offsets [O0, O1) of the face's stencil.
-template <bool J, int O0 = 0, int O1 = NOFF>
+// One thread per face of rows [j0, j1], i = i0, i0 + istep, ...: J = false
+// i-faces (O4: rows j >= 3, the compact viscous flux; else rows 1, 2), J =
+// true j-faces (ROW 0: rows j >= 4, centred Euler flux; ROW 3, 2: the
+// off-centred rows 3 (compact viscous flux) and 2 (2nd-order)); offsets
+// [O0, O1) of the face's stencil.
+template <bool J, bool O4, int ROW, int O0 = 0, int O1 = NOFF>
 __global__ void __launch_bounds__(256) k_face_reverse(Mesh m, Phys ph, Metric g, const double* __restrict__ w,
                                                       const double* __restrict__ prim,
                                                       const double* __restrict__ grad, double* __restrict__ out,
@@ -506,7 +525,7 @@ __global__ void __launch_bounds__(256) k_face_reverse(Mesh m, Phys ph, Metric g,
   const long f = J ? (i - 1) + (long)m.im * (j - 1) : (i - 1) + (long)(m.im + 1) * (j - 1);
   double hb[5];
   face_adjoint(m, resb, i, j, J ? 0 : 1, J ? 1 : 0, hb);
-  face_reverse<J, O0, O1>(m, ph, g, w, prim, grad, i, j, hb, out, nf, f);
+  face_reverse<J, O4, ROW, O0, O1>(m, ph, g, w, prim, grad, i, j, hb, out, nf, f);
 }
 
 }  // namespace bca

@@ -25,13 +25,11 @@ void launch_jface_blocks_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_blocks_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_iface_vjp_o4(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_iface_vjp_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_vjp_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_iface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_iface_vjp_rev_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 void launch_jface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_rev_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+void launch_jface_vjp_rev_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
 
 }  // namespace bca

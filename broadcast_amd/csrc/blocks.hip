@@ -1,8 +1,8 @@
 // One face-kernel variant per compilation (selected by BCA_VARIANT): the 24
 // seeded face instantiations of each variant are large, so the build compiles
 // the variants as separate objects in parallel.  Variants 0-4 store Jacobian
-// blocks, 5-9 the VJP contributions of the same passes, 10 the reverse-mode
-// face adjoints of the interior faces.
+// blocks, 5 the VJP contributions of the wall faces by the same passes, 6 the
+// reverse-mode face adjoints of the other faces.
 #include <hip/hip_runtime.h>
 
 #include "adjface.h"
@@ -47,21 +47,15 @@ BCA_LAUNCH(launch_jface_blocks_r3, NOFF, k_jface_blocks, 3, MODE_BLOCKS)
 BCA_LAUNCH(launch_jface_blocks_r2, NOFF, k_jface_blocks, 2, MODE_BLOCKS)
 BCA_LAUNCH(launch_jface_blocks_wall, NOFF, k_jface_blocks, 1, MODE_BLOCKS)
 #elif BCA_VARIANT == 5
-BCA_LAUNCH(launch_iface_vjp_o4, OFF_GROUP, k_iface_blocks, true, MODE_VJP)
-#elif BCA_VARIANT == 6
-BCA_LAUNCH(launch_iface_vjp_o2, NOFF, k_iface_blocks, false, MODE_VJP)
-#elif BCA_VARIANT == 7
-BCA_LAUNCH(launch_jface_vjp_c, OFF_GROUP, k_jface_blocks, 0, MODE_VJP)
-#elif BCA_VARIANT == 8
-BCA_LAUNCH(launch_jface_vjp_r3, NOFF, k_jface_blocks, 3, MODE_VJP)
-#elif BCA_VARIANT == 9
-BCA_LAUNCH(launch_jface_vjp_r2, NOFF, k_jface_blocks, 2, MODE_VJP)
 BCA_LAUNCH(launch_jface_vjp_wall, NOFF, k_jface_blocks, 1, MODE_VJP)
-#elif BCA_VARIANT == 10
-BCA_LAUNCH(launch_iface_vjp_rev, REV_GROUP, k_face_reverse, false)
-BCA_LAUNCH(launch_jface_vjp_rev, REV_GROUP, k_face_reverse, true)
+#elif BCA_VARIANT == 6
+BCA_LAUNCH(launch_iface_vjp_rev, REV_GROUP, k_face_reverse, false, true, 0)
+BCA_LAUNCH(launch_iface_vjp_rev_o2, REV_GROUP, k_face_reverse, false, false, 0)
+BCA_LAUNCH(launch_jface_vjp_rev, REV_GROUP, k_face_reverse, true, true, 0)
+BCA_LAUNCH(launch_jface_vjp_rev_r3, REV_GROUP, k_face_reverse, true, true, 3)
+BCA_LAUNCH(launch_jface_vjp_rev_r2, REV_GROUP, k_face_reverse, true, false, 2)
 #else
-#error "BCA_VARIANT must be 0..10"
+#error "BCA_VARIANT must be 0..6"
 #endif
 
 }  // namespace bca

@@ -1072,12 +1072,12 @@ int bca_vjp_dnc5(bca_ctx* c, double* residub, double* wb, const double* w, bca_m
   }
   BCA_LAUNCHED();
   BlockArgs ae{m, ph, gm, w, c->w0.prim, c->w0.grad, c->vjpi, residub, 1, 2};
-  launch_iface_vjp_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
+  launch_iface_vjp_rev_o2(grid2(m.im + 1, 2, b), b, es[0], ae);
   ae.out = c->vjpj;
   ae.j0 = ae.j1 = 3;
-  launch_jface_vjp_r3(grid2(m.im, 1, b), b, es[1], ae);
+  launch_jface_vjp_rev_r3(grid2(m.im, 1, b), b, es[1], ae);
   ae.j0 = ae.j1 = 2;
-  launch_jface_vjp_r2(grid2(m.im, 1, b), b, es[2], ae);
+  launch_jface_vjp_rev_r2(grid2(m.im, 1, b), b, es[2], ae);
   ae.j0 = ae.j1 = 1;
   launch_jface_vjp_wall(grid2(m.im, 1, b), b, es[0], ae);
   BCA_LAUNCHED();
