@@ -1528,7 +1528,7 @@ int jac_csr(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* p, const
     c->ghostG_cap = gsize;
   }
   {
-    Timed t(c, "k_assemble_ghosts");
+    Timed t(c, "k_ghost_part");
     k_ghost_part<<<dim3(grid1(near_count(m), 256), nslot), 256, 0, s>>>(m, z, c->blki, c->blkj, c->ghostT,
                                                                       c->ghostnz, c->ghostG);
   }
