@@ -9,7 +9,9 @@
 // forward-mode line function composes).  So the transpose is formed from the
 // tangent line functions: for each offset d, a unit Dual<5> tangent on the
 // offset-d cell of every line (lines are disjoint), the tangent BC applied,
-// and the ghost tangents contracted with wb(ghosts).
+// and the ghost tangents contracted with wb(ghosts).  The four offsets are
+// four slots of one seeded array (slot d at ws + d cs, blockIdx.y = d), so a
+// BC takes one seed, one tangent BC and one accumulate launch.
 #pragma once
 #include "bcs.h"
 #include "jacobian.h"
@@ -30,8 +32,11 @@ __device__ inline void line_cell(int loc, Box b, int l, int& i, int& j, int& i0,
 // cell of every line of the interface, written on the frame the boundary
 // conditions read or write only (the ghost frame plus 4 interior layers,
 // frame_cell(m, BC_FRAME, t); the offset-d cells, d <= 3, lie in it).
-__global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ ws, int loc, Box b, int d) {
+__global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ wsd, int loc, Box b,
+                           long cs) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  Dual<5>* __restrict__ ws = wsd + d * cs;
   if (t >= frame_count(m, BC_FRAME)) return;
   int a, bb;
   frame_cell(m, BC_FRAME, t, a, bb);
@@ -50,9 +55,11 @@ __global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __rest
 
 // wb(offset-d cell, q) += sum over the line's ghosts g and variables e of
 // d ws(g, e) / d w(offset-d cell, q) * wb(g, e)
-__global__ void k_adj_accumulate(Mesh m, const Dual<5>* __restrict__ ws, double* __restrict__ wb, int loc, Box b,
-                                 int d, int lmax) {
+__global__ void k_adj_accumulate(Mesh m, const Dual<5>* __restrict__ wsd, double* __restrict__ wb, int loc, Box b,
+                                 long cs, int lmax) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  const Dual<5>* __restrict__ ws = wsd + d * cs;
   if (l > lmax) return;
   int i, j, i0, j0;
   line_cell(loc, b, l, i, j, i0, j0);

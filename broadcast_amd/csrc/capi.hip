@@ -1472,15 +1472,17 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
     return 0;
   }
   if ((rc = frame_check(m, *op))) return rc;
-  if ((rc = alloc_planes(c, &c->wadj, 5 * m.ncell))) return rc;
+  // the offsets d = 0..3 in ADJ_REACH slots of wadj (only the BC frame of
+  // each is written or read)
+  const long cs = 5 * m.ncell;
+  if ((rc = alloc_planes(c, &c->wadj, ADJ_REACH * cs))) return rc;
   const int lmax = (int)bc_items(*op);
-  for (int d = 0; d < ADJ_REACH; ++d) {
-    k_adj_seed<<<grid1(frame_count(m, BC_FRAME), 256), 256, 0, c->stream>>>(m, w, c->wadj, op->loc, b, d);
-    BCA_LAUNCHED();
-    if ((rc = launch_bc<Dual<5>>(c, c->wadj, *op, nx, ny))) return rc;
-    k_adj_accumulate<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, c->wadj, wb, op->loc, b, d, lmax);
-    BCA_LAUNCHED();
-  }
+  k_adj_seed<<<dim3(grid1(frame_count(m, BC_FRAME), 256), ADJ_REACH), 256, 0, c->stream>>>(m, w, c->wadj, op->loc,
+                                                                                          b, cs);
+  BCA_LAUNCHED();
+  if ((rc = launch_bc<Dual<5>>(c, c->wadj, *op, nx, ny, ADJ_REACH, cs))) return rc;
+  k_adj_accumulate<<<dim3(grid1(lmax, 64), ADJ_REACH), 64, 0, c->stream>>>(m, c->wadj, wb, op->loc, b, cs, lmax);
+  BCA_LAUNCHED();
   k_adj_clear_ghosts<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, wb, op->loc, b, lmax);
   BCA_LAUNCHED();
   return 0;
