@@ -338,15 +338,12 @@ bool box_in_frame(const Mesh& m, int i0, int j0, int i1, int j1, int W) {
 // line and 3 cells further in, ADJ_REACH) and every ghost it writes lie in
 // that frame: the interface sits on the block edge, join boxes lie in the
 // frame.
-int frame_check(const Mesh& m, const bca_bc_op& op) {
+bool op_in_frame(const Mesh& m, const bca_bc_op& op) {
   const int W = BC_FRAME;
   const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
   if (op.kind == BCA_BC_JOIN) {
     const Box d{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
-    if (!box_in_frame(m, b.imin, b.jmin, b.imax, b.jmax, W) || !box_in_frame(m, d.imin, d.jmin, d.imax, d.jmax, W))
-      return fail(-1, "join boxes must lie in the boundary frame (ghost layers + 4 interior layers) for the "
-                      "linearised/adjoint BC sequence of the colored Jacobian and BC adjoints");
-    return 0;
+    return box_in_frame(m, b.imin, b.jmin, b.imax, b.jmax, W) && box_in_frame(m, d.imin, d.jmin, d.imax, d.jmax, W);
   }
   const Side s = decode_side(op.loc, b.imin, b.jmin, b.imax, b.jmax);
   const int in = ADJ_REACH - 1;  // deepest interior offset read
@@ -355,10 +352,46 @@ int frame_check(const Mesh& m, const bca_bc_op& op) {
   if (s.i0 < 0) { i0 -= in; i1 += m.gh; }
   if (s.j0 > 0) { j0 -= m.gh; j1 += in; }
   if (s.j0 < 0) { j0 -= in; j1 += m.gh; }
-  if (!box_in_frame(m, i0, j0, i1, j1, W))
-    return fail(-1, "interface not on the block edge: the linearised/adjoint BC sequence of the colored "
-                    "Jacobian and BC adjoints covers the boundary frame (ghost layers + 4 interior layers) only");
-  return 0;
+  return box_in_frame(m, i0, j0, i1, j1, W);
+}
+int frame_check(const Mesh& m, const bca_bc_op& op) {
+  if (op_in_frame(m, op)) return 0;
+  if (op.kind == BCA_BC_JOIN)
+    return fail(-1, "join boxes must lie in the boundary frame (ghost layers + 4 interior layers) for the "
+                    "linearised/adjoint BC sequence of the colored Jacobian and BC adjoints");
+  return fail(-1, "interface not on the block edge: the linearised/adjoint BC sequence of the colored "
+                  "Jacobian and BC adjoints covers the boundary frame (ghost layers + 4 interior layers) only");
+}
+
+// (w, wd) <-> Dual<1> on the boundary frame only (frame_cell(m, BC_FRAME, t)),
+// for a tangent BC whose reads and writes lie in it (op_in_frame)
+__global__ void k_pack_frame(Mesh m, const double* __restrict__ a, const double* __restrict__ ad,
+                             Dual<1>* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= frame_count(m, BC_FRAME)) return;
+  int aa, bb;
+  frame_cell(m, BC_FRAME, t, aa, bb);
+  const long c = aa + (long)m.ni * bb;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    Dual<1> x(a[c + e * m.ncell]);
+    x.d[0] = ad[c + e * m.ncell];
+    out[c + e * m.ncell] = x;
+  }
+}
+__global__ void k_unpack_frame(Mesh m, const Dual<1>* __restrict__ x, double* __restrict__ a,
+                               double* __restrict__ ad) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= frame_count(m, BC_FRAME)) return;
+  int aa, bb;
+  frame_cell(m, BC_FRAME, t, aa, bb);
+  const long c = aa + (long)m.ni * bb;
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const Dual<1> v = x[c + e * m.ncell];
+    a[c + e * m.ncell] = v.v;
+    ad[c + e * m.ncell] = v.d[0];
+  }
 }
 
 // bc_check + frame_check of a whole BC sequence, before anything is launched
@@ -685,6 +718,18 @@ static int bc_entry(bca_ctx* c, double* w, double* wd, const bca_bc_op& op, cons
   if (!wd) return launch_bc<double>(c, w, op, nx, ny);
   int rc = ensure_work(c, c->w1, true);
   if (rc) return rc;
+  if ((rc = bc_check(c->m, op, nx, ny))) return rc;
+  // a BC on the block edge reads and writes the boundary frame only: pack
+  // and unpack that (1/50 of the array at 2048 x 1024), else the whole array
+  if (op_in_frame(c->m, op)) {
+    const unsigned nb = grid1(frame_count(c->m, BC_FRAME), 256);
+    k_pack_frame<<<nb, 256, 0, c->stream>>>(c->m, w, wd, c->w1.w);
+    BCA_LAUNCHED();
+    if ((rc = launch_bc<Dual<1>>(c, c->w1.w, op, nx, ny))) return rc;
+    k_unpack_frame<<<nb, 256, 0, c->stream>>>(c->m, c->w1.w, w, wd);
+    BCA_LAUNCHED();
+    return 0;
+  }
   const long n = 5 * c->m.ncell;
   k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
   BCA_LAUNCHED();
