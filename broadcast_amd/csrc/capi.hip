@@ -653,10 +653,14 @@ static int jvp_impl(bca_ctx* c, double* residu, double* residud, const double* w
   const long n = 5 * m.ncell;
   k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
   BCA_LAUNCHED();
-  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, c->w1.res, to_metric(g), to_phys(p), nowall, ord)))
-    return rc;
+  // the pipeline up to the face fluxes, then the balance straight into
+  // residu / residud
+  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, nullptr, to_metric(g), to_phys(p), nowall, ord))) return rc;
   const dim3 b(64, 4);
-  k_unpack_interior<1><<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w1.res, residu, residud);
+  {
+    Timed t(c, "k_balance<" + std::string(tname<Dual<1>>()) + ">");
+    k_balance_split<1><<<grid2(m.im, m.jm, b), b, 0, c->stream>>>(m, c->w1.hn, residu, residud);
+  }
   BCA_LAUNCHED();
   return 0;
 }

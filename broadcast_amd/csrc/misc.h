@@ -366,19 +366,4 @@ __global__ void k_unpack(long n, const Dual<N>* __restrict__ x, double* __restri
   if (ad) ad[c] = x[c].d[0];
 }
 
-// residu and residud on the interior only (the reference leaves ghosts untouched)
-template <int N>
-__global__ void k_unpack_interior(Mesh m, const Dual<N>* __restrict__ x, double* __restrict__ a,
-                                  double* __restrict__ ad) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i > m.im || j > m.jm) return;
-  const long c = cidx(m, i, j);
-  for (int e = 0; e < 5; ++e) {
-    const Dual<N> v = x[c + e * m.ncell];
-    if (a) a[c + e * m.ncell] = v.v;
-    if (ad) ad[c + e * m.ncell] = v.d[0];
-  }
-}
-
 }  // namespace bca

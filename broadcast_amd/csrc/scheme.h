@@ -298,4 +298,21 @@ __global__ void k_balance(Mesh m, const T* __restrict__ hn, T* __restrict__ resi
   for (int e = 0; e < 5; ++e) residu[c + (long)e * m.ncell] = balance_at(m, hn, i, j, e);
 }
 
+// The balance of Dual<N> fluxes written as residu (values) and residud (the
+// first tangent) on the interior (either may be null; ghosts untouched).
+template <int N>
+__global__ void k_balance_split(Mesh m, const Dual<N>* __restrict__ hn, double* __restrict__ residu,
+                                double* __restrict__ residud) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > m.im || j > m.jm) return;
+  const long c = cidx(m, i, j);
+#pragma unroll
+  for (int e = 0; e < 5; ++e) {
+    const Dual<N> v = balance_at(m, hn, i, j, e);
+    if (residu) residu[c + (long)e * m.ncell] = v.v;
+    if (residud) residud[c + (long)e * m.ncell] = v.d[0];
+  }
+}
+
 }  // namespace bca
