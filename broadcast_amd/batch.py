@@ -1,11 +1,12 @@
-"""Residual + JVP of a batch of meshes of different sizes in one launch.
+"""Residual + JVP of a batch of meshes of different sizes, launched together.
 
 Per mesh the work is the reference's per-iteration sequence -- the BC calls in
 order, then flux_num_dnc5_2d (BROADCAST_npz.py:1018-1031) or, with a tangent,
 the linearised BCs and flux_num_dnc5_2d_d (:1079-1085) -- as ``Flow.residual``
-/ ``Flow.jvp`` do one mesh at a time; ``UnevenBatch`` runs all meshes in one
-persistent cooperative kernel (bca_batch_residual_jvp), so small meshes do not
-each pay a chain of launches on a mostly idle chip.
+/ ``Flow.jvp`` do one mesh at a time; ``UnevenBatch`` runs all meshes
+together, one launch per stage over the concatenated work of every mesh
+(bca_batch_residual_jvp), so small meshes do not each pay a chain of
+launches on a mostly idle chip.
 """
 from __future__ import annotations
 

@@ -1,5 +1,5 @@
-// Residual (+ JVP) of a batch of meshes of different sizes in one persistent
-// cooperative launch.
+// Residual (+ JVP) of a batch of meshes of different sizes, one launch per
+// stage over all meshes.
 //
 // Per mesh this is the reference's per-iteration sequence -- the BC calls in
 // order (BROADCAST_npz.py:1018-1021 primal, :1079-1082 tangent) then
@@ -7,11 +7,12 @@
 // srcfv/tangent/flux_num_dnc5_d.f90:15) -- with the same device functions as
 // the per-mesh kernels.  A batch of small and uneven meshes would leave most
 // of the chip idle under one launch per stage and mesh (a 128 x 64 mesh is 32
-// blocks of 256 cells on 256 CUs); here every stage is one grid-stride loop
-// over the concatenated work of all meshes, the stages separated by grid
-// barriers, so the whole batch is ~10 stage passes of a resident grid.
+// blocks of 256 cells on 256 CUs); here every stage is one launch over the
+// concatenated work of all meshes (one thread per item, the mesh of an item
+// found in the stage's prefix table), so the whole batch is ~10 launches,
+// each at its own occupancy.
 //
-// Stages (each a loop over the flat index of all meshes' items):
+// Stages (each over the flat index of all meshes' items):
 //   pack      every cell        ws = (w, wd) as Dual<1> (or w as double)
 //   bc k      lines of op k     the k-th BC of every mesh that has one
 //   prim      every cell        primitives
@@ -21,8 +22,6 @@
 //   balance   interior cells    residu (and residud); ghost frame of ws back
 //                               into w (and wd), as the BC calls leave them
 #pragma once
-#include <hip/hip_cooperative_groups.h>
-
 #include "bcs.h"
 #include "faces.h"
 
@@ -149,34 +148,31 @@ __device__ inline void batch_ghost_grad(const BatchItem<T>& it, int t) {
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_batch(const BatchItem<T>* __restrict__ items,
-                                               const BatchPlan* __restrict__ plan) {
-  namespace cg = cooperative_groups;
-  cg::grid_group grid = cg::this_grid();
-  const long stride = (long)gridDim.x * blockDim.x;
-  const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// One kernel per stage, one thread per item t of the stage (items of all
+// meshes concatenated in the order of the prefix table).
+enum { BS_PACK = 0, BS_BC, BS_PRIM, BS_GRAD, BS_GHOSTGRAD, BS_FACE, BS_BALANCE };
+
+template <class T, int STAGE>
+__global__ void __launch_bounds__(256) k_batch_stage(const BatchItem<T>* __restrict__ items,
+                                                     const BatchPlan* __restrict__ plan, int op) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int n = plan->n;
-  for (long t = t0; t < plan->cells[n]; t += stride) {
+  if constexpr (STAGE == BS_PACK) {
+    if (t >= plan->cells[n]) return;
     const int k = batch_which(plan->cells, n, t);
     batch_pack(items[k], t - plan->cells[k]);
-  }
-  for (int b = 0; b < plan->maxops; ++b) {
-    grid.sync();
-    const long* pre = plan->bc[b];
-    for (long t = t0; t < pre[n]; t += stride) {
-      const int k = batch_which(pre, n, t);
-      batch_bc(items[k], items[k].ops[b], (int)(t - pre[k]));
-    }
-  }
-  grid.sync();
-  for (long t = t0; t < plan->cells[n]; t += stride) {
+  } else if constexpr (STAGE == BS_BC) {
+    const long* pre = plan->bc[op];
+    if (t >= pre[n]) return;
+    const int k = batch_which(pre, n, t);
+    batch_bc(items[k], items[k].ops[op], (int)(t - pre[k]));
+  } else if constexpr (STAGE == BS_PRIM) {
+    if (t >= plan->cells[n]) return;
     const int k = batch_which(plan->cells, n, t);
     const BatchItem<T>& it = items[k];
     primitives(it.m, it.ph, it.ws, it.prim, t - plan->cells[k]);
-  }
-  grid.sync();
-  for (long t = t0; t < plan->interior[n]; t += stride) {
+  } else if constexpr (STAGE == BS_GRAD) {
+    if (t >= plan->interior[n]) return;
     const int k = batch_which(plan->interior, n, t);
     const BatchItem<T>& it = items[k];
     const int q = (int)(t - plan->interior[k]);
@@ -186,21 +182,18 @@ __global__ void __launch_bounds__(256) k_batch(const BatchItem<T>* __restrict__ 
     const long c = cidx(it.m, i, j);
 #pragma unroll
     for (int g = 0; g < 4; ++g) it.grad[c + g * it.m.ncell] = out[g];
-  }
-  grid.sync();
-  for (long t = t0; t < plan->edges[n]; t += stride) {
+  } else if constexpr (STAGE == BS_GHOSTGRAD) {
+    if (t >= plan->edges[n]) return;
     const int k = batch_which(plan->edges, n, t);
     batch_ghost_grad(items[k], (int)(t - plan->edges[k]));
-  }
-  grid.sync();
-  for (long t = t0; t < plan->faces[n]; t += stride) {
+  } else if constexpr (STAGE == BS_FACE) {
+    if (t >= plan->faces[n]) return;
     const int k = batch_which(plan->faces, n, t);
     const BatchItem<T>& it = items[k];
     const int q = (int)(t - plan->faces[k]);
     batch_face(it, 1 + q % (it.m.im + 1), 1 + q / (it.m.im + 1));
-  }
-  grid.sync();
-  for (long t = t0; t < plan->cells[n]; t += stride) {
+  } else {
+    if (t >= plan->cells[n]) return;
     const int k = batch_which(plan->cells, n, t);
     const BatchItem<T>& it = items[k];
     const Mesh& m = it.m;

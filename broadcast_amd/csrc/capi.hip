@@ -1417,7 +1417,6 @@ int run_batch(bca_batch* b, const bca_batch_item* items, int n) {
   T* p = reinterpret_cast<T*>(b->scratch);
   plan->n = n;
   plan->maxops = 0;
-  long work = 0;
   for (int k = 0; k < n; ++k) {
     const bca_batch_item& it = items[k];
     const Mesh& m = ms[k];
@@ -1442,7 +1441,6 @@ int run_batch(bca_batch* b, const bca_batch_item* items, int n) {
     plan->interior[k + 1] = plan->interior[k] + (long)m.im * m.jm;
     plan->edges[k + 1] = plan->edges[k] + m.im + m.jm;
     plan->faces[k + 1] = plan->faces[k] + (long)(m.im + 1) * (m.jm + 1);
-    work = std::max(work, plan->faces[k + 1]);
   }
   for (int o = 0; o < BATCH_OPS; ++o)
     for (int k = 0; k < n; ++k)
@@ -1451,18 +1449,27 @@ int run_batch(bca_batch* b, const bca_batch_item* items, int n) {
   if (!b->copied) BCA_HIP(hipEventCreateWithFlags(&b->copied, hipEventDisableTiming));
   BCA_HIP(hipEventRecord(b->copied, b->stream));
   b->pending = true;
-  // a resident grid: blocks per CU at this kernel's occupancy times the CUs,
-  // no more than the largest stage needs
-  int dev = 0, ncu = 0, per_cu = 0;
-  BCA_HIP(hipGetDevice(&dev));
-  BCA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  BCA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<T>, 256, 0));
-  if (per_cu < 1) return fail(-2, "batch kernel does not fit on a CU");
-  const unsigned grid = (unsigned)std::max(1L, std::min((long)per_cu * ncu, (work + 255) / 256));
+  // one launch per stage over all meshes' items
   const BatchItem<T>* ditems = reinterpret_cast<const BatchItem<T>*>(b->table);
   const BatchPlan* dplan = reinterpret_cast<const BatchPlan*>(ditems + BATCH_MAX);
-  void* args[] = {(void*)&ditems, (void*)&dplan};
-  BCA_HIP(hipLaunchCooperativeKernel((const void*)k_batch<T>, dim3(grid), dim3(256), args, 0, b->stream));
+  hipStream_t s = b->stream;
+  k_batch_stage<T, BS_PACK><<<grid1(plan->cells[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
+  for (int o = 0; o < plan->maxops; ++o) {
+    if (plan->bc[o][n] == 0) continue;
+    k_batch_stage<T, BS_BC><<<grid1(plan->bc[o][n], 256), 256, 0, s>>>(ditems, dplan, o);
+    BCA_LAUNCHED();
+  }
+  k_batch_stage<T, BS_PRIM><<<grid1(plan->cells[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
+  k_batch_stage<T, BS_GRAD><<<grid1(plan->interior[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
+  k_batch_stage<T, BS_GHOSTGRAD><<<grid1(plan->edges[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
+  k_batch_stage<T, BS_FACE><<<grid1(plan->faces[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
+  k_batch_stage<T, BS_BALANCE><<<grid1(plan->cells[n], 256), 256, 0, s>>>(ditems, dplan, 0);
+  BCA_LAUNCHED();
   return 0;
 }
 

@@ -301,7 +301,7 @@ int bca_jacobian_colored_2zones(bca_ctx *ctx, const double *w, bca_metrics g, co
  * (BROADCAST_npz.py:1018-1021, tangent :1079-1082) then flux_num_dnc5_2d
  * (srcfv/rhs/flux_num_dnc5.F90:7-226) or flux_num_dnc5_2d_d
  * (srcfv/tangent/flux_num_dnc5_d.f90:15) -- for up to 16 meshes of different
- * sizes (gh = 3) in one persistent cooperative launch.  Per item: the ghost
+ * sizes (gh = 3), one launch per stage over all meshes.  Per item: the ghost
  * frames of w (and wd) are filled in place by the BC sequence, residu (and
  * residud = dR/dw . wd) are written on the interior.  Items carry wd and
  * residud all or none (residual only).  No reference counterpart as a batch:
