@@ -1,4 +1,4 @@
-"""Uneven-mesh batch (one persistent launch for the BCs + residual + JVP of
+"""Uneven-mesh batch (one launch per stage for the BCs + residual + JVP of
 several meshes) against the per-mesh path and the oracle.
 
 The per-mesh path (Flow.apply_bcs + bca_jvp_dnc5) is itself pinned against
