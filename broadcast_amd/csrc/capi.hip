@@ -1336,13 +1336,13 @@ int bca_jbeta_fill(bca_ctx* c, const double* jac, const double* jdz, const doubl
     c->jbeta_cap = nnz;
   }
   const Mesh& m = c->m;
-  const dim3 b(64, 4);
-  dim3 gr = grid2(m.im, m.jm, b);
-  gr.z = 5;
   {
+    // the tiled CSR fill in its three-array mode: indices and the pruned
+    // components (jac / vol(row), dz, dz2) on the pattern
     Timed t(c, "k_jbeta_parts");
-    k_jbeta_parts<<<gr, b, 0, c->stream>>>(m, jac, jdz, jdz2, vol, mini, c->csr_mask, indptr, indices, c->jbeta_comp,
-                                           nnz);
+    const dim3 gr((m.im + FILL_TI - 1) / FILL_TI, m.jm, 5);
+    k_csr_fill<1, true><<<gr, FILL_THREADS, 0, c->stream>>>(m, 0, jac, mini, c->csr_mask, indptr, indices,
+                                                            c->jbeta_comp, jdz, jdz2, vol, nnz);
   }
   BCA_LAUNCHED();
   {

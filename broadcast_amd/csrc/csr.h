@@ -181,11 +181,14 @@ __global__ void __launch_bounds__(256) k_csr_count(Mesh m, int withjn, A jac, do
 constexpr int FILL_TI = 16;        // rows (consecutive i) per block
 constexpr int FILL_THREADS = 512;  // 8 waves, FILL_TI / 8 rows each
 
-template <int NZ>
+template <int NZ, bool JB = false>
 __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, const double* __restrict__ jac,
                                                           double mini, const unsigned long long* __restrict__ masks,
                                                           const int* __restrict__ indptr, int* __restrict__ indices,
-                                                          double* __restrict__ data) {
+                                                          double* __restrict__ data,
+                                                          const double* __restrict__ jdz = nullptr,
+                                                          const double* __restrict__ jdz2 = nullptr,
+                                                          const double* __restrict__ vol = nullptr, long nnz = 0) {
   constexpr int NS = 245 * NZ;          // slots per row: (k, l, m, zone)
   constexpr int NA = 7 * NZ;            // distinct window columns in i, at most
   constexpr int PITCH = FILL_TI + 1;    // odd pitch: lanes reading different slots spread over the banks
@@ -201,11 +204,14 @@ __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, c
   const int im = m.im, jm = m.jm;
   const long plane = (long)im * jm;
   const long nt = 5L * im * jm;
+#pragma unroll 1
+  for (int c = 0; c < (JB ? 3 : 1); ++c) {
   {
     // slot values: thread (s0, il) loads planes s0, s0 + 32, ...
     const int il = tid % FILL_TI, s0 = tid / FILL_TI;
     const bool in = i0 + il <= im;
-    const double* src = jac + (i0 - 1 + il) + (long)(j - 1) * im + (long)e * plane;
+    const double* arr = c == 0 ? jac : c == 1 ? jdz : jdz2;
+    const double* src = arr + (i0 - 1 + il) + (long)(j - 1) * im + (long)e * plane;
     constexpr int STEP = FILL_THREADS / FILL_TI;
     constexpr int NL = (NS + STEP - 1) / STEP;
     double v[NL];
@@ -214,7 +220,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, c
       const int s = s0 + q * STEP;
       v[q] = (in && s < NS) ? src[5L * s * plane] : 0.0;
     }
-    if (tid < FILL_TI) {
+    if (c == 0 && tid < FILL_TI) {
       const int i = i0 + tid;
       if (i <= im) {
         RowOrder<NZ> o;
@@ -240,7 +246,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, c
         start_s[tid] = indptr[e + (j - 1) * 5 + (i - 1) * jm * 5];
       }
     }
-    if (tid < CSR_MASK_WORDS * FILL_TI) {
+    if (c == 0 && tid < CSR_MASK_WORDS * FILL_TI) {
       const int w = tid / FILL_TI, r = tid % FILL_TI;
       if (i0 + r <= im) mk_s[w][r] = masks[w * nt + csr_tid(m, i0 + r, j, e)];
     }
@@ -255,6 +261,7 @@ __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, c
   for (int r = wave; r < FILL_TI; r += FILL_THREADS / 64) {
     if (i0 + r > im) break;
     int pos = start_s[r];
+    const double v0 = JB ? vol[cidx(m, i0 + r, j)] : 1.0;
 #pragma unroll
     for (int w = 0; w < CSR_MASK_WORDS; ++w) {
       const unsigned long long mk = mk_s[w][r];
@@ -273,11 +280,15 @@ __global__ void __launch_bounds__(FILL_THREADS) k_csr_fill(Mesh m, int withjn, c
           }
         }
         const int p = pos + __popcll(mk & ((1ull << lane) - 1ull));
-        indices[p] = mm + col_b[b] + col_a[r][a];
-        data[p] = val;
+        if (c == 0) indices[p] = mm + col_b[b] + col_a[r][a];
+        if constexpr (JB) data[c * nnz + p] = c == 0 ? (val != 0.0 ? val / v0 : 0.0) : val;
+        else data[p] = val;
       }
       pos += __popcll(mk);
     }
+  }
+  // the next array's tile overwrites this one
+  if constexpr (JB) __syncthreads();
   }
 }
 

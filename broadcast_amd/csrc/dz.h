@@ -383,37 +383,9 @@ struct MaxAbs3 {
   __device__ double operator[](long s) const { return fmax(fabs(a[s]), fmax(fabs(b[s]), fabs(c[s]))); }
 };
 
-// Pass 1: the column indices and the three pruned components on the pattern,
-// comp[p] = (jac/vol(row), dz, dz2) (zero where pruned).
-__global__ void __launch_bounds__(256) k_jbeta_parts(Mesh m, const double* __restrict__ jac,
-                                                     const double* __restrict__ jdz, const double* __restrict__ jdz2,
-                                                     const double* __restrict__ vol, double mini,
-                                                     const unsigned long long* __restrict__ masks,
-                                                     const int* __restrict__ indptr, int* __restrict__ indices,
-                                                     double* __restrict__ comp, long nnz) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int e = blockIdx.z;
-  if (i > m.im || j > m.jm) return;
-  RowOrder<1> o;
-  row_order<1>(m, 0, i, j, o);
-  unsigned long long mk[CSR_MASK_WORDS];
-  const long tid = csr_tid(m, i, j, e), nt = 5L * m.im * m.jm;
-#pragma unroll
-  for (int w = 0; w < CSR_MASK_WORDS; ++w) mk[w] = masks[w * nt + tid];
-  const int start = indptr[e + (j - 1) * 5 + (i - 1) * m.jm * 5];
-  const double v0 = vol[cidx(m, i, j)];
-  const MaxAbs3 mv{jac, jdz, jdz2};
-  row_scan_slots<1>(m, o, mv, i, j, e, [&](int t, long col, double v, long slot) {
-    if (!(v > mini)) return;
-    const int pos = start + mask_rank(mk, t);
-    indices[pos] = (int)col;
-    const double x = jac[slot], y = jdz[slot], y2 = jdz2[slot];
-    comp[pos] = fabs(x) > mini ? x / v0 : 0.0;
-    comp[nnz + pos] = fabs(y) > mini ? y : 0.0;
-    comp[2 * nnz + pos] = fabs(y2) > mini ? y2 : 0.0;
-  });
-}
+// Pass 1 is k_csr_fill<1, true> (csr.h): the column indices and the three
+// pruned components on the pattern, comp[p] = (jac/vol(row), dz, dz2) (zero
+// where pruned), from LDS tiles of the three COOs.
 
 // Pass 2, element-wise and coalesced: vals[b][p] = (jv - beta_b^2 dz2, beta_b dz).
 __global__ void __launch_bounds__(256) k_jbeta_values(const double* __restrict__ comp, long nnz,
