@@ -660,3 +660,44 @@ def test_full_size_cylinder_csr_applies_the_jacobian(torch_mod):
     fused = f.jacobian_csr(w, coefdiag=cd)
     for a, b, what in zip(fused, (ip2, ix2, d2), ("indptr", "indices", "data")):
         assert t.equal(a, b), what
+
+
+@pytest.mark.parametrize("kind", ["extrap", "wall"])
+def test_tangent_bc_frame_and_full_array_packing(torch_mod, O, kind):
+    """A tangent BC whose box lies on the block edge packs and unpacks the
+    boundary frame only; one whose box lies inside the block (column im-6 or
+    row 6 of a 24 x 16 block) takes the whole-array pack.  Both give the
+    oracle's tangent BC on the same arrays, and leave every cell the BC does
+    not write untouched."""
+    from broadcast_amd.srcfv import f_lin
+
+    case = bl_case(24, 16, name="bl24x16")
+    im, jm, gh = case.im, case.jm, case.gh
+    rng = np.random.default_rng(11)
+    shape = (im + 2 * gh, jm + 2 * gh, 5)
+    w0 = np.asfortranarray(1.0 + 0.1 * rng.random(shape))
+    w0[..., 4] += 2.5  # positive pressure for the wall's power law
+    wd0 = np.asfortranarray(rng.standard_normal(shape))
+    if kind == "extrap":
+        loc, boxes = "Ihi", [case.interf["Ihi"], np.array([[im - 6, 1], [im - 6, jm]], dtype=float, order="F")]
+    else:
+        loc, boxes = "Jlo", [case.interf["Jlo"], np.array([[1, 6], [im, 6]], dtype=float, order="F")]
+    for box in boxes:
+        w, wd = w0.copy(order="F"), wd0.copy(order="F")
+        if kind == "extrap":
+            f_lin.bc_extrapolate_o2_2d_d(w, wd, loc, box, im, jm, gh)
+        else:
+            f_lin.bc_wall_viscous_adia_2d_d(w, wd, loc, case.phys.gam, box, gh, im, jm)
+        wr, wdr = w0.copy(order="F"), wd0.copy(order="F")
+        if kind == "extrap":
+            O.bc_extrapolate_o2_2d(wr, loc, box, im, jm, gh, wd=wdr)
+        else:
+            O.bc_wall_viscous_adia_2d(wr, loc, case.phys.gam, box, gh, im, jm, wd=wdr)
+        assert not np.array_equal(wr, w0), "the BC wrote nothing"
+        ok, err = close(w, wr, 1e-14)
+        assert ok, (box.tolist(), err)
+        ok, err = close(wd, wdr, 1e-13)
+        assert ok, (box.tolist(), err)
+        untouched = (wr == w0) & (wdr == wd0)
+        assert np.array_equal(w[untouched], w0[untouched]), box.tolist()
+        assert np.array_equal(wd[untouched], wd0[untouched]), box.tolist()
