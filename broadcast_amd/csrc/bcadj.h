@@ -29,9 +29,9 @@ __device__ inline void line_cell(int loc, Box b, int l, int& i, int& j, int& i0,
 }
 
 // ws = w as Dual<5> with unit tangents d[e] on variable e of the offset-d
-// cell of every line of the interface, written on the frame the boundary
-// conditions read or write only (the ghost frame plus 4 interior layers,
-// frame_cell(m, BC_FRAME, t); the offset-d cells, d <= 3, lie in it).
+// cell of every line of the interface, stored as the compact frame the
+// boundary conditions read or write (the ghost frame plus 4 interior layers,
+// StateView<T, true> layout; the offset-d cells, d <= 3, lie in it).
 __global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __restrict__ wsd, int loc, Box b,
                            long cs) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,7 +40,7 @@ __global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __rest
   if (t >= frame_count(m, BC_FRAME)) return;
   int a, bb;
   frame_cell(m, BC_FRAME, t, a, bb);
-  const long c = a + (long)m.ni * bb;
+  const long c = a + (long)m.ni * bb, fc = frame_count(m, BC_FRAME);
   const int i = a - m.gh + 1, j = bb - m.gh + 1;
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
   const bool on = s.kdir == 1 ? (i == b.imin + d * s.i0 && j >= b.jmin && j <= b.jmax)
@@ -49,7 +49,7 @@ __global__ void k_adj_seed(Mesh m, const double* __restrict__ w, Dual<5>* __rest
   for (int e = 0; e < 5; ++e) {
     Dual<5> x(w[c + e * m.ncell]);
     if (on) x.d[e] = 1.0;
-    ws[c + e * m.ncell] = x;
+    ws[t + e * fc] = x;
   }
 }
 
@@ -63,13 +63,14 @@ __global__ void k_adj_accumulate(Mesh m, const Dual<5>* __restrict__ wsd, double
   if (l > lmax) return;
   int i, j, i0, j0;
   line_cell(loc, b, l, i, j, i0, j0);
-  const long n = m.ncell;
+  const long n = m.ncell, fc = frame_count(m, BC_FRAME);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int de = 1; de <= m.gh; ++de) {
-    const long g = cidx(m, i - de * i0, j - de * j0);
+    const int ig = i - de * i0, jg = j - de * j0;
+    const long g = cidx(m, ig, jg), gf = frame_index(m, BC_FRAME, ig + m.gh - 1, jg + m.gh - 1);
 #pragma unroll
     for (int e = 0; e < 5; ++e) {
-      const Dual<5> x = ws[g + e * n];
+      const Dual<5> x = ws[gf + e * fc];
       const double y = wb[g + e * n];
 #pragma unroll
       for (int q = 0; q < 5; ++q) acc[q] += x.d[q] * y;

@@ -35,22 +35,62 @@ struct Box {
   int imin, jmin, imax, jmax;
 };
 
-template <class T>
+// ------------------------------------------------------------------ frames
+// Storage coordinates (a, b) of the t-th cell of the frame of width W around
+// the block (all cells within W of the array edge): rows b < W, rows
+// b >= nj - W, then the left/right W columns of the rows in between.  A block
+// too small for a frame of that width is taken whole, row by row.
+__host__ __device__ inline bool frame_whole(const Mesh& m, int W) { return 2 * W >= m.ni || 2 * W >= m.nj; }
+__host__ __device__ inline long frame_count(const Mesh& m, int W) {
+  if (frame_whole(m, W)) return m.ncell;
+  return 2L * W * m.ni + 2L * W * (m.nj - 2 * W);
+}
+__device__ inline void frame_cell(const Mesh& m, int W, long t, int& a, int& b) {
+  if (frame_whole(m, W) || t < (long)W * m.ni) { b = (int)(t / m.ni); a = (int)(t % m.ni); return; }
+  t -= (long)W * m.ni;
+  if (t < (long)W * m.ni) { b = m.nj - W + (int)(t / m.ni); a = (int)(t % m.ni); return; }
+  t -= (long)W * m.ni;
+  b = W + (int)(t / (2 * W));
+  const int s = (int)(t % (2 * W));
+  a = s < W ? s : m.ni - 2 * W + s;
+}
+// the inverse: frame position t of the storage cell (a, b), which must lie in
+// the frame
+__host__ __device__ inline long frame_index(const Mesh& m, int W, int a, int b) {
+  if (frame_whole(m, W) || b < W) return a + (long)m.ni * b;
+  if (b >= m.nj - W) return (long)W * m.ni + a + (long)m.ni * (b - (m.nj - W));
+  return 2L * W * m.ni + 2L * W * (b - W) + (a < W ? a : a - (m.ni - 2 * W));
+}
+
+// Frame read or written by the boundary conditions of a box on the block
+// edge: the ghost layers plus 4 interior layers.
+constexpr int BC_FRAME = 7;
+
+// Element (i, j, e) of a state stored as the whole array (FR false: planes
+// of ncell cells) or as the compact boundary frame of width BC_FRAME (FR
+// true: planes of frame_count cells; for the seeded Dual<5> states that only
+// the boundary conditions touch).
+template <class T, bool FR = false>
 struct StateView {
   Mesh m;
   T* w;
-  __device__ T& operator()(int i, int j, int e) const { return w[cidx(m, i, j) + (long)e * m.ncell]; }
+  long plane;
+  __device__ StateView(const Mesh& mm, T* ww) : m(mm), w(ww), plane(FR ? frame_count(mm, BC_FRAME) : mm.ncell) {}
+  __device__ T& operator()(int i, int j, int e) const {
+    const long c = FR ? frame_index(m, BC_FRAME, i + m.gh - 1, j + m.gh - 1) : cidx(m, i, j);
+    return w[c + (long)e * plane];
+  }
 };
 
 // Characteristic sub/supersonic inflow.  Kept quirks of the reference:
 // the Roe-state pressure lacks the 1/2 (:49), velo2 uses ww unsquared (:69),
 // the ghost spanwise momentum is field(j, de, 4) (:160).
-template <class T>
+template <class T, bool FR = false>
 __device__ inline void bc_inlet_line(const Mesh& m, T* __restrict__ wp, int loc, Box b,
                                      const double* __restrict__ field, int lm, const double* __restrict__ nxp,
                                      const double* __restrict__ nyp, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  StateView<T> w{m, wp};
+  StateView<T, FR> w{m, wp};
   auto F = [&](int a, int d, int e) { return field[(a - 1) + (long)lm * ((d - 1) + (long)m.gh * e)]; };
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
   const int j = b.jmin + (l - 1) * s.i0 * s.i0;
@@ -136,22 +176,22 @@ __device__ inline void bc_inlet_line(const Mesh& m, T* __restrict__ wp, int loc,
   }
 }
 
-template <class T>
+template <class T, bool FR = false>
 __global__ void k_bc_inlet(Mesh m, T* __restrict__ wbase, int loc, Box b, const double* __restrict__ field,
                            int lm, const double* __restrict__ nxp, const double* __restrict__ nyp,
                            double gam, long cstride) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
-    bc_inlet_line(m, wbase + (long)blockIdx.y * cstride, loc, b, field, lm, nxp, nyp, gam, l);
+    bc_inlet_line<T, FR>(m, wbase + (long)blockIdx.y * cstride, loc, b, field, lm, nxp, nyp, gam, l);
 }
 
 // Non-reflecting far field against the reference state wbd(l, :).
-template <class T>
+template <class T, bool FR = false>
 __device__ inline void bc_noref_line(const Mesh& m, T* __restrict__ wp, const double* __restrict__ wbd, int lm,
                                      int loc, Box b, const double* __restrict__ nxp,
                                      const double* __restrict__ nyp, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  StateView<T> w{m, wp};
+  StateView<T, FR> w{m, wp};
   const int i1 = s.i0 * s.i0, j1 = s.j0 * s.j0;
   const int i = b.imin + (l - 1) * j1;
   const int j = b.jmin + (l - 1) * i1;
@@ -208,19 +248,19 @@ __device__ inline void bc_noref_line(const Mesh& m, T* __restrict__ wp, const do
   }
 }
 
-template <class T>
+template <class T, bool FR = false>
 __global__ void k_bc_noref(Mesh m, T* __restrict__ wbase, const double* __restrict__ wbd, int lm, int loc,
                            Box b, const double* __restrict__ nxp, const double* __restrict__ nyp,
                            double gam, long cstride) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
-    bc_noref_line(m, wbase + (long)blockIdx.y * cstride, wbd, lm, loc, b, nxp, nyp, gam, l);
+    bc_noref_line<T, FR>(m, wbase + (long)blockIdx.y * cstride, wbd, lm, loc, b, nxp, nyp, gam, l);
 }
 
-template <class T>
+template <class T, bool FR = false>
 __device__ inline void bc_extrap2_line(const Mesh& m, T* __restrict__ wp, int loc, Box b, int em, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  StateView<T> w{m, wp};
+  StateView<T, FR> w{m, wp};
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
   const int j = b.jmin + (l - 1) * s.i0 * s.i0;
   for (int de = 1; de <= m.gh; ++de)
@@ -229,20 +269,20 @@ __device__ inline void bc_extrap2_line(const Mesh& m, T* __restrict__ wp, int lo
           2.0 * w(i - s.i0 * (de - 1), j - s.j0 * (de - 1), e) - w(i - s.i0 * (de - 2), j - s.j0 * (de - 2), e);
 }
 
-template <class T>
+template <class T, bool FR = false>
 __global__ void k_bc_extrap2(Mesh m, T* __restrict__ wbase, int loc, Box b, int em, long cstride) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
-    bc_extrap2_line(m, wbase + (long)blockIdx.y * cstride, loc, b, em, l);
+    bc_extrap2_line<T, FR>(m, wbase + (long)blockIdx.y * cstride, loc, b, em, l);
 }
 
 // Adiabatic no-slip wall: first ghost from the 2nd-order zero-normal-gradient
 // pressure and the isentropic density, the deeper ones mirror interior cells
 // with reversed velocity.
-template <class T>
+template <class T, bool FR = false>
 __device__ inline void bc_wall_line(const Mesh& m, T* __restrict__ wp, int loc, Box b, double gam, int l) {
   const Side s = decode_side(loc, b.imin, b.jmin, b.imax, b.jmax);
-  StateView<T> w{m, wp};
+  StateView<T, FR> w{m, wp};
   const int i = b.imin + (l - 1) * s.j0 * s.j0;
   const int j = b.jmin + (l - 1) * s.i0 * s.i0;
   const double g1 = gam - 1.0;
@@ -276,11 +316,11 @@ __device__ inline void bc_wall_line(const Mesh& m, T* __restrict__ wp, int loc, 
   }
 }
 
-template <class T>
+template <class T, bool FR = false>
 __global__ void k_bc_wall(Mesh m, T* __restrict__ wbase, int loc, Box b, double gam, long cstride) {
   const int l = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   if (l <= decode_side(loc, b.imin, b.jmin, b.imax, b.jmax).lmax)
-    bc_wall_line(m, wbase + (long)blockIdx.y * cstride, loc, b, gam, l);
+    bc_wall_line<T, FR>(m, wbase + (long)blockIdx.y * cstride, loc, b, gam, l);
 }
 
 // Block copy between two boxes of one block (the periodic join of the O-mesh).
@@ -292,7 +332,7 @@ __host__ __device__ inline int join_count(Box prd, int tr0, int tr1) {
   return (phi[idir - 1] - plo[idir - 1] + 1) * (phi[jdir - 1] - plo[jdir - 1] + 1);
 }
 
-template <class T>
+template <class T, bool FR = false>
 __device__ inline void join_cell(const Mesh& m, T* __restrict__ wp, Box prr, Box prd, int tr0, int tr1, int em,
                                  int ni_arr, long plane, int t) {
   const int istep = tr0 >= 0 ? 1 : -1, jstep = tr1 >= 0 ? 1 : -1;
@@ -305,18 +345,18 @@ __device__ inline void join_cell(const Mesh& m, T* __restrict__ wp, Box prr, Box
   const int jj = jstep == 1 ? 1 + bq : nj_ - bq;
   const int ir = prr.imin + a, jr = prr.jmin + bq;
   const int id = plo[idir - 1] + ii - 1, jd = plo[jdir - 1] + jj - 1;
-  const long off = m.gh - 1;
-  const long rdst = (ir + off) + (long)ni_arr * (jr + off);
-  const long rsrc = (id + off) + (long)ni_arr * (jd + off);
+  const int off = m.gh - 1;
+  const long rdst = FR ? frame_index(m, BC_FRAME, ir + off, jr + off) : (ir + off) + (long)ni_arr * (jr + off);
+  const long rsrc = FR ? frame_index(m, BC_FRAME, id + off, jd + off) : (id + off) + (long)ni_arr * (jd + off);
   for (int e = 0; e < em; ++e) wp[rdst + e * plane] = wp[rsrc + e * plane];
 }
 
-template <class T>
+template <class T, bool FR = false>
 __global__ void k_join(Mesh m, T* __restrict__ wbase, Box prr, Box prd, int tr0, int tr1, int em,
                        int ni_arr, int nj_arr, long plane, long cstride) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < join_count(prd, tr0, tr1))
-    join_cell(m, wbase + (long)blockIdx.y * cstride, prr, prd, tr0, tr1, em, ni_arr, plane, t);
+    join_cell<T, FR>(m, wbase + (long)blockIdx.y * cstride, prr, prd, tr0, tr1, em, ni_arr, plane, t);
 }
 
 }  // namespace bca

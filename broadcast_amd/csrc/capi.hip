@@ -98,7 +98,7 @@ struct bca_ctx {
   double* blki = nullptr;   // face-local Jacobian blocks, i-faces, one plane per offset
   double* blkj = nullptr;   // j-faces
   double* ghostT = nullptr; // ghost tangents per zone and color
-  Dual<5>* wcol = nullptr;  // seeded states of the zones x 49 colors (frames only)
+  Dual<5>* wcol = nullptr;  // seeded states of the zones x 49 colors (compact BC frames)
   unsigned char* ghostnz = nullptr;  // nonzero flags of the ghost tangents
   int zone_cap = 0;         // number of zones the three arrays above hold
   int* csr_cnt = nullptr;   // row lengths of the pruned CSR
@@ -108,7 +108,7 @@ struct bca_ctx {
   double* vjpj = nullptr;   // j-faces
   double* jbeta_comp = nullptr;  // J(beta) components on the pattern
   long jbeta_cap = 0;
-  Dual<5>* wadj = nullptr;  // seeded state of the BC adjoints
+  Dual<5>* wadj = nullptr;  // seeded states of the BC adjoints (compact BC frames)
   double* ghostG = nullptr; // ghost part of the fused CSR path, per near row and slot color
   long ghostG_cap = 0;
   CsrRecord csr_rec;        // the last pruned CSR written without the COO (refresh key)
@@ -410,7 +410,8 @@ long bc_items(const bca_bc_op& op) {
   return decode_side(op.loc, b.imin, b.jmin, b.imax, b.jmax).lmax;
 }
 
-template <class T>
+// FR: w holds ncolor states in the compact BC_FRAME layout (StateView<T, true>)
+template <class T, bool FR = false>
 int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const double* ny, int ncolor = 1,
               long cstride = 0) {
   const Mesh& m = c->m;
@@ -421,21 +422,21 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
   const unsigned nb = grid1(bc_items(op), op.kind == BCA_BC_JOIN ? 256 : 64);
   switch (op.kind) {
     case BCA_BC_INLET:
-      k_bc_inlet<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny, op.gam, cstride);
+      k_bc_inlet<T, FR><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.data, op.lm, nx, ny, op.gam, cstride);
       break;
     case BCA_BC_NOREF:
-      k_bc_noref<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny, op.gam, cstride);
+      k_bc_noref<T, FR><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.data, op.lm, op.loc, b, nx, ny, op.gam, cstride);
       break;
     case BCA_BC_EXTRAP2:
-      k_bc_extrap2<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, 5, cstride);
+      k_bc_extrap2<T, FR><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, 5, cstride);
       break;
     case BCA_BC_WALL_ADIA:
-      k_bc_wall<T><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.gam, cstride);
+      k_bc_wall<T, FR><<<dim3(nb, ncolor), 64, 0, s>>>(m, w, op.loc, b, op.gam, cstride);
       break;
     default: {
       const Box prd{op.box2[0], op.box2[1], op.box2[2], op.box2[3]};
-      k_join<T><<<dim3(nb, ncolor), 256, 0, s>>>(m, w, b, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj, m.ncell,
-                                                  cstride);
+      k_join<T, FR><<<dim3(nb, ncolor), 256, 0, s>>>(m, w, b, prd, op.tr[0], op.tr[1], 5, m.ni, m.nj,
+                                                      FR ? frame_count(m, BC_FRAME) : m.ncell, cstride);
       break;
     }
   }
@@ -890,19 +891,20 @@ int ghost_tangents(bca_ctx* c, const double* w, const bca_metrics& g, const bca_
   const long ng = ghost_count(m);
   int rc = 0;
   if (c->zone_cap < z.n) {
-    c->bytes -= (long)c->zone_cap * ncol * (ng * 25 * (long)sizeof(double) + 5 * m.ncell * (long)sizeof(Dual<5>) + ng);
+    c->bytes -= (long)c->zone_cap * ncol *
+                (ng * 25 * (long)sizeof(double) + 5 * frame_count(m, BC_FRAME) * (long)sizeof(Dual<5>) + ng);
     if ((rc = realloc_planes(c, &c->ghostT, (long)nslot * ng * 25))) return rc;
-    if ((rc = realloc_planes(c, &c->wcol, (long)nslot * 5 * m.ncell))) return rc;
+    if ((rc = realloc_planes(c, &c->wcol, (long)nslot * 5 * frame_count(m, BC_FRAME)))) return rc;
     if ((rc = realloc_planes(c, &c->ghostnz, (long)nslot * ng))) return rc;
     c->zone_cap = z.n;
   }
   hipStream_t s = c->stream;
   Timed t(c, "ghost_tangents");
-  const long cs = 5 * m.ncell;
+  const long cs = 5 * frame_count(m, BC_FRAME);
   k_seed_frames<<<dim3(grid1(frame_count(m, BC_FRAME), 256), nslot), 256, 0, s>>>(m, z, w, c->wcol, cs);
   BCA_LAUNCHED();
   for (int q = 0; q < nops; ++q)
-    if ((rc = launch_bc<Dual<5>>(c, c->wcol, ops[q], g.nx, g.ny, nslot, cs))) return rc;
+    if ((rc = launch_bc<Dual<5>, true>(c, c->wcol, ops[q], g.nx, g.ny, nslot, cs))) return rc;
   k_ghost_tangents<<<dim3(grid1(ng, 256), nslot), 256, 0, s>>>(m, c->wcol, cs, c->ghostT, c->ghostnz);
   BCA_LAUNCHED();
   return 0;
@@ -1528,15 +1530,15 @@ int bca_bc_adjoint(bca_ctx* c, const double* w, double* wb, const bca_bc_op* op,
     return 0;
   }
   if ((rc = frame_check(m, *op))) return rc;
-  // the offsets d = 0..3 in ADJ_REACH slots of wadj (only the BC frame of
-  // each is written or read)
-  const long cs = 5 * m.ncell;
+  // the offsets d = 0..3 in ADJ_REACH slots of wadj, each the compact BC
+  // frame
+  const long cs = 5 * frame_count(m, BC_FRAME);
   if ((rc = alloc_planes(c, &c->wadj, ADJ_REACH * cs))) return rc;
   const int lmax = (int)bc_items(*op);
   k_adj_seed<<<dim3(grid1(frame_count(m, BC_FRAME), 256), ADJ_REACH), 256, 0, c->stream>>>(m, w, c->wadj, op->loc,
                                                                                           b, cs);
   BCA_LAUNCHED();
-  if ((rc = launch_bc<Dual<5>>(c, c->wadj, *op, nx, ny, ADJ_REACH, cs))) return rc;
+  if ((rc = launch_bc<Dual<5>, true>(c, c->wadj, *op, nx, ny, ADJ_REACH, cs))) return rc;
   k_adj_accumulate<<<dim3(grid1(lmax, 64), ADJ_REACH), 64, 0, c->stream>>>(m, c->wadj, wb, op->loc, b, cs, lmax);
   BCA_LAUNCHED();
   k_adj_clear_ghosts<<<grid1(lmax, 64), 64, 0, c->stream>>>(m, wb, op->loc, b, lmax);

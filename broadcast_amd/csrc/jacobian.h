@@ -23,6 +23,7 @@
 // For every seed this is exactly the reference's JVP (the tangent of a face is
 // the sum over its active inputs), so the COO values agree to rounding.
 #pragma once
+#include "bcs.h"
 #include "blockskern.h"
 #include "misc.h"
 
@@ -48,30 +49,11 @@ __device__ inline bool zone_column(const Mesh& m, const Zones& z, int zn, int i,
 }
 
 // ------------------------------------------------------------------ ghosts
-// Storage coordinates (a, b) of the t-th cell of the frame of width W around
-// the block (all cells within W of the array edge): rows b < W, rows
-// b >= nj - W, then the left/right W columns of the rows in between.  A block
-// too small for a frame of that width is taken whole, row by row.
-__host__ __device__ inline bool frame_whole(const Mesh& m, int W) { return 2 * W >= m.ni || 2 * W >= m.nj; }
-__host__ __device__ inline long frame_count(const Mesh& m, int W) {
-  if (frame_whole(m, W)) return m.ncell;
-  return 2L * W * m.ni + 2L * W * (m.nj - 2 * W);
-}
-__device__ inline void frame_cell(const Mesh& m, int W, long t, int& a, int& b) {
-  if (frame_whole(m, W) || t < (long)W * m.ni) { b = (int)(t / m.ni); a = (int)(t % m.ni); return; }
-  t -= (long)W * m.ni;
-  if (t < (long)W * m.ni) { b = m.nj - W + (int)(t / m.ni); a = (int)(t % m.ni); return; }
-  t -= (long)W * m.ni;
-  b = W + (int)(t / (2 * W));
-  const int s = (int)(t % (2 * W));
-  a = s < W ? s : m.ni - 2 * W + s;
-}
-
 // Seeded states of all zones x 49 colors at once (blockIdx.y = zone * 49 +
-// color), written only on the frame the boundary conditions read or write:
-// the ghost frame plus 4 interior layers.  State values from w, unit tangent
+// color), stored as the compact frame the boundary conditions read or write
+// (the ghost frame plus 4 interior layers; StateView<T, true> layout, planes
+// of frame_count(m, BC_FRAME) cells).  State values from w, unit tangent
 // d[m] on variable m of the cells the color seeds in the zone, zero elsewhere.
-constexpr int BC_FRAME = 7;
 __global__ void k_seed_frames(Mesh m, Zones z, const double* __restrict__ w, Dual<5>* __restrict__ ws,
                               long cstride) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -85,13 +67,13 @@ __global__ void k_seed_frames(Mesh m, Zones z, const double* __restrict__ w, Dua
   const int i0 = z.lo[zn];
   const bool on = i >= i0 + l && i <= z.hi[zn] && j >= k + 1 && j <= m.jm && (i - i0 - l) % step == 0 &&
                   (j - k - 1) % step == 0;
-  const long c = a + (long)m.ni * b;
+  const long c = a + (long)m.ni * b, fc = frame_count(m, BC_FRAME);
   Dual<5>* out = ws + (long)blockIdx.y * cstride;
 #pragma unroll
   for (int e = 0; e < 5; ++e) {
     Dual<5> x(w[c + e * m.ncell]);
     if (on) x.d[e] = 1.0;
-    out[c + e * m.ncell] = x;
+    out[t + e * fc] = x;
   }
 }
 
@@ -107,13 +89,13 @@ __global__ void k_ghost_tangents(Mesh m, const Dual<5>* __restrict__ ws, long cs
   const int color = blockIdx.y;
   int a, b;
   frame_cell(m, m.gh, g, a, b);  // the ghost frame, in ghost_index order
-  const long c = a + (long)m.ni * b;
+  const long c = frame_index(m, BC_FRAME, a, b), fc = frame_count(m, BC_FRAME);
   const Dual<5>* src = ws + (long)color * cstride;
   double* out = T + ((long)color * ng + g) * 25;
   bool any = false;
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
-    const Dual<5> x = src[c + q * m.ncell];
+    const Dual<5> x = src[c + q * fc];
 #pragma unroll
     for (int mm = 0; mm < 5; ++mm) {
       out[q * 5 + mm] = x.d[mm];
