@@ -216,15 +216,22 @@ int need_gh3(const bca_ctx* c) {
 }
 
 // The residual pipeline on scalar type T; residu == nullptr stops after the faces.
-template <class T>
-int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g, const Phys& ph,
+// WP: the state as planes of T, or a SplitState (T = Dual<1>)
+template <class T, class WP = const T*>
+int run_pipeline(bca_ctx* c, Work<T>& wk, WP w, T* residu, const Metric& g, const Phys& ph,
                  bool nowall = false, int ord = 5) {
   const Mesh& m = c->m;
   hipStream_t s = c->stream;
   const std::string sfx = std::string("<") + tname<T>() + ">";
+  // the faces read the state by stencil from planes of T: a SplitState is
+  // copied to wk.w by k_prim on the way
+  const T* wf;
+  T* wcopy = nullptr;
+  if constexpr (std::is_same<WP, SplitState>::value) wf = wcopy = wk.w;
+  else wf = w;
   {
     Timed t(c, "k_prim" + sfx);
-    k_prim<T><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, wk.prim);
+    k_prim<T, WP><<<grid1(m.ncell, 256), 256, 0, s>>>(m, ph, w, wk.prim, wcopy);
   }
   BCA_LAUNCHED();
   const dim3 b(64, 4);
@@ -243,16 +250,16 @@ int run_pipeline(bca_ctx* c, Work<T>& wk, const T* w, T* residu, const Metric& g
   {
     Timed t(c, "k_face" + sfx);
     const dim3 gf = grid2(m.im + 1, m.jm + 1, bf);
-#define BCA_FACE(NW, ORD) k_face<T, NW, ORD><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn)
+#define BCA_FACE(NW, ORD) k_face<T, NW, ORD><<<gf, bf, 0, s>>>(m, ph, g, wf, wk.prim, wk.grad, wk.hn)
     if (ord == 3 && nowall) BCA_FACE(true, 3);
     else if (ord == 3) BCA_FACE(false, 3);
     else if (ord == 7 && nowall) BCA_FACE(true, 7);
     else if (ord == 7) BCA_FACE(false, 7);
     else if (ord == 9 && nowall) BCA_FACE(true, 9);
     else if (ord == 9) BCA_FACE(false, 9);
+    else if (nowall) BCA_FACE(true, 5);
+    else BCA_FACE(false, 5);
 #undef BCA_FACE
-    else if (nowall) k_face<T, true><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
-    else k_face<T><<<gf, bf, 0, s>>>(m, ph, g, w, wk.prim, wk.grad, wk.hn);
   }
   BCA_LAUNCHED();
   if (residu) {
@@ -651,12 +658,12 @@ static int jvp_impl(bca_ctx* c, double* residu, double* residud, const double* w
   if (rc) return rc;
   if ((rc = ensure_work(c, c->w1, true))) return rc;
   const Mesh& m = c->m;
-  const long n = 5 * m.ncell;
-  k_pack<1><<<grid1(n, 256), 256, 0, c->stream>>>(n, w, wd, c->w1.w);
-  BCA_LAUNCHED();
-  // the pipeline up to the face fluxes, then the balance straight into
+  // the pipeline up to the face fluxes, reading (w, wd) as Dual<1> where it
+  // reads the state (no packed copy), then the balance straight into
   // residu / residud
-  if ((rc = run_pipeline<Dual<1>>(c, c->w1, c->w1.w, nullptr, to_metric(g), to_phys(p), nowall, ord))) return rc;
+  if ((rc = run_pipeline<Dual<1>, SplitState>(c, c->w1, SplitState{w, wd}, nullptr, to_metric(g), to_phys(p),
+                                               nowall, ord)))
+    return rc;
   const dim3 b(64, 4);
   {
     Timed t(c, "k_balance<" + std::string(tname<Dual<1>>()) + ">");

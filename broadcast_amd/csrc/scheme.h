@@ -154,9 +154,22 @@ struct Metric {
   const double* volf; // 2 cell planes
 };
 
+// The state (w, wd) of a tangent read straight from its two double arrays as
+// Dual<1> values, in place of a packed Dual<1> copy: w[k] = (w[k], wd[k]).
+struct SplitState {
+  const double* v;
+  const double* d;
+  __device__ Dual<1> operator[](long k) const {
+    Dual<1> x(v[k]);
+    x.d[0] = d[k];
+    return x;
+  }
+};
+
 // ---------------------------------------------------------------- primitives
-template <class T>
-__device__ inline void primitives(const Mesh& m, const Phys& ph, const T* w, T* prim, long c) {
+// WP: a pointer to planes of T, or a SplitState for T = Dual<1>
+template <class T, class WP = const T*>
+__device__ inline void primitives(const Mesh& m, const Phys& ph, WP w, T* prim, long c) {
   const double cvm1 = 1.0 / ph.cv;
   const double betas = ph.muref * (ph.tref + ph.cs) / (sqrt(ph.tref) * ph.tref);
   const long n = m.ncell;
@@ -178,10 +191,16 @@ __device__ inline void primitives(const Mesh& m, const Phys& ph, const T* w, T* 
   prim[c + P_MU * n] = betas / (t + ph.s_suth) * dsqrt(t) * t;
 }
 
-template <class T>
-__global__ void k_prim(Mesh m, Phys ph, const T* __restrict__ w, T* __restrict__ prim) {
+// wcopy (optional): the state as planes of T as well, for the kernels that
+// read it by stencil (one load per value there, where a SplitState takes two)
+template <class T, class WP = const T*>
+__global__ void k_prim(Mesh m, Phys ph, WP w, T* __restrict__ prim, T* __restrict__ wcopy = nullptr) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < m.ncell) primitives(m, ph, w, prim, c);
+  if (c >= m.ncell) return;
+  primitives<T, WP>(m, ph, w, prim, c);
+  if (wcopy)
+#pragma unroll
+    for (int e = 0; e < 5; ++e) wcopy[c + e * m.ncell] = w[c + e * m.ncell];
 }
 
 // ---------------------------------------------------------------- gradients
