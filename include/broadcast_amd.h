@@ -283,7 +283,10 @@ int bca_jbeta_fill(bca_ctx *ctx, const double *jac, const double *jdz, const dou
  * computejacobianfromjv[_relaxed][_withjn], into the same COO arrays the
  * reference loop fills (a fresh, zeroed COO).  No dense tangent is formed:
  * face-local Jacobian blocks are computed once and summed per row and seed.
- * write_index = 0 skips ia/ja (they depend on the mesh only). */
+ * write_index = 0 skips ia/ja (they depend on the mesh only).  Every BC box
+ * of `ops` must lie on the block edge and every join box in the 7-layer
+ * boundary frame; the seeded states are stored for that frame only, and any
+ * other box returns an error. */
 int bca_jacobian_colored(bca_ctx *ctx, const double *w, bca_metrics g, const bca_phys *phys,
                          const bca_bc_op *ops, int nops, const double *coefdiag, int withjn,
                          double *jac, int *ia, int *ja, long nbentry, int write_index);
@@ -333,7 +336,9 @@ int bca_batch_residual_jvp(bca_batch *batch, const bca_batch_item *items, int n)
  * one block: wb(prd) += wb(prr), wb(prr) = 0 -- the transpose of the copy
  * (the reference routine takes two arrays and zeroes the donor adjoint
  * first, which in one block would clear wb).  nx, ny: the metrics (inlet and
- * no-reflexion read them). */
+ * no-reflexion read them).  Except for the join, the box must lie on the
+ * block edge (the seeded states cover the 7-layer boundary frame only);
+ * other boxes return an error. */
 int bca_bc_adjoint(bca_ctx *ctx, const double *w, double *wb, const bca_bc_op *op, const double *nx,
                    const double *ny);
 
