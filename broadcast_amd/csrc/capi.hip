@@ -426,6 +426,8 @@ int launch_bc(bca_ctx* c, T* w, const bca_bc_op& op, const double* nx, const dou
   const Box b{op.box[0], op.box[1], op.box[2], op.box[3]};
   int rc = bc_check(m, op, nx, ny);
   if (rc) return rc;
+  // a compact-frame state holds no cell outside the frame
+  if (FR && (rc = frame_check(m, op))) return rc;
   const unsigned nb = grid1(bc_items(op), op.kind == BCA_BC_JOIN ? 256 : 64);
   switch (op.kind) {
     case BCA_BC_INLET:
