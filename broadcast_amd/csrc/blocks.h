@@ -19,17 +19,19 @@ struct BlockArgs {
   int i0 = 1, istep = 1;  // faces i = i0 + k istep (k = 0, 1, ...) of each row
 };
 
-void launch_iface_blocks_o4(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_iface_blocks_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_blocks_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_blocks_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_blocks_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_blocks_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_iface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_iface_vjp_rev_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_rev_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
-void launch_jface_vjp_rev_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a);
+// s2 (optional): every second offset group goes to s2 instead of s (the
+// groups write planes of their own)
+void launch_iface_blocks_o4(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_iface_blocks_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_blocks_c(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_blocks_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_blocks_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_blocks_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_vjp_wall(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_iface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_iface_vjp_rev_o2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_vjp_rev(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_vjp_rev_r3(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
+void launch_jface_vjp_rev_r2(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2 = nullptr);
 
 }  // namespace bca

@@ -27,11 +27,12 @@ void for_groups(L&& launch) {
 // one launch, where 24 launches of a few blocks each would cost more in
 // launch latency than the spills do.
 #define BCA_LAUNCH(name, GS, kern, ...)                                                  \
-  void name(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a) {                       \
+  void name(dim3 gr, dim3 bl, hipStream_t s, const BlockArgs& a, hipStream_t s2) {       \
     static_assert(NOFF % (GS) == 0, "offset groups");                                    \
     for_groups<GS>([&](auto O) {                                                         \
       constexpr int o0 = decltype(O)::value;                                             \
-      kern<__VA_ARGS__, o0, o0 + (GS)><<<gr, bl, 0, s>>>(BCA_ARGS);                      \
+      hipStream_t st = (s2 && (o0 / (GS)) % 2) ? s2 : s;                                 \
+      kern<__VA_ARGS__, o0, o0 + (GS)><<<gr, bl, 0, st>>>(BCA_ARGS);                     \
     });                                                                                  \
   }
 

@@ -1021,13 +1021,18 @@ int jacobian_phase1(bca_ctx* c, const double* w, bca_metrics g, const bca_phys* 
   BlockArgs ai{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blki, nullptr, 3, m.jm};
   const dim3 bt = c->tune_blocks;
   {
+    // every second offset on side stream 0 (free once its edge rows are
+    // done); the timers cover the offsets on c->stream
     Timed t(c, "k_iface_blocks");
-    launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, bt), bt, s, ai);
+    launch_iface_blocks_o4(grid2(m.im + 1, m.jm - 2, bt), bt, s, ai, es[0] == s ? nullptr : es[0]);
   }
   BlockArgs aj{m, ph, gm, w, c->w0.prim, c->w0.grad, c->blkj, nullptr, 4, m.jm + 1};
   {
+    // likewise every second j-face offset on side stream 2: the one-offset
+    // launches (tens of microseconds each) leave most of the chip idle
+    // alone, so four of them run side by side
     Timed t(c, "k_jface_blocks");
-    launch_jface_blocks_c(grid2(m.im, m.jm - 2, bt), bt, s, aj);
+    launch_jface_blocks_c(grid2(m.im, m.jm - 2, bt), bt, s, aj, es[2] == s ? nullptr : es[2]);
   }
   BCA_LAUNCHED();
   return edge_join(c, es);
